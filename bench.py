@@ -1,0 +1,218 @@
+"""Benchmark of the tomosynthesis reconstruction hot path on one MI355X.
+
+A step is one MLEM iteration of the reference discretisation (recons::MLEM,
+KevLoq/Kevernals Reconstructors.h:182-242) on the full-size workload:
+1024x1024x64 voxels (0.15x0.15x1 mm), 2048^2 detector (0.15 mm), 25 views
+on +-20 deg, fused into two kernels (forward projection + ratio,
+backprojection + update).  Inputs are resident in HBM before timing starts.
+
+Output: one JSON line.  value = GRays/s forward+back = views*nu*nv / t(step).
+roofline: the dominant kernel's algorithmic bytes per launch (accepted
+(ray, voxel) pairs * 4 B + its streamed arrays) over its average duration,
+measured with HIP events on the stream the kernels run on.  cpu_baseline: the
+CPU oracle (a port of the reference arithmetic, OpenMP) timed on a bounded
+sample of the same workload on this host, extrapolated to one full step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time to RMSE≤1% vs converged MLEM, learned vs plain; GRays/s fwd+back"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_data(K, ctx, g, torch, seed=2):
+    """Seeded phantom with 200 microcalcifications; Poisson transmission data
+    at I0 = 5e3; reference-discretisation data p = FP_R(mu) * l_hat / l."""
+    from kevernals_amd.phantom import phantom, poisson_line_integrals
+
+    t = time.time()
+    mu_h = phantom(g, seed, microcalcifications=200)
+    mu = torch.from_numpy(mu_h).cuda()
+    ell = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(mu, ell, K.MODE_A)
+    torch.cuda.synchronize()
+    ell_hat = torch.from_numpy(poisson_line_integrals(ell.cpu().numpy(), 5e3, seed + 1000)).cuda()
+    fp = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(mu, fp, K.MODE_R)
+    data = torch.where(ell > 0, fp * ell_hat / torch.clamp(ell, min=1e-30), torch.zeros_like(fp))
+    torch.cuda.synchronize()
+    log(f"[bench] data ready in {time.time() - t:.1f} s")
+    return mu, data
+
+
+def cpu_baseline(g, mu_h, data_h, seconds):
+    """Oracle FP+BP (mode R, OpenMP over rays / voxels) on a sample spread
+    over all views and slices, extrapolated to one full forward+back pass."""
+    from oracle.oracle import Oracle
+
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(g, nt)
+    rng = np.random.Generator(np.random.PCG64(7))
+    # calibrate with a small sample, then size the sample to ~seconds
+    rows_per_view = 2
+    t = time.time()
+    for p in range(g.nviews):
+        v0 = int(rng.integers(0, g.nv - rows_per_view))
+        o.fp_r_rows(mu_h, p, v0, v0 + rows_per_view)
+    t_fp_ray = (time.time() - t) / (g.nviews * rows_per_view * g.nu)
+    vox = rng.integers(0, g.nvox, size=2000)
+    t = time.time()
+    o.bp_r_voxels(data_h, vox)
+    t_bp_vox = (time.time() - t) / len(vox)
+    budget = max(seconds, 1.0)
+    est_fp = t_fp_ray * g.nrays
+    est_bp = t_bp_vox * g.nvox
+    share_fp = est_fp / (est_fp + est_bp)
+    n_rows = max(1, int(budget * share_fp / (t_fp_ray * g.nu * g.nviews)))
+    n_vox = max(1000, int(budget * (1 - share_fp) / t_bp_vox))
+    t = time.time()
+    for p in range(g.nviews):
+        v0 = int(rng.integers(0, max(1, g.nv - n_rows)))
+        o.fp_r_rows(mu_h, p, v0, min(g.nv, v0 + n_rows))
+    fp_s = time.time() - t
+    rays_done = g.nviews * min(n_rows, g.nv) * g.nu
+    vox = rng.integers(0, g.nvox, size=n_vox)
+    t = time.time()
+    o.bp_r_voxels(data_h, vox)
+    bp_s = time.time() - t
+    full_s = fp_s / rays_done * g.nrays + bp_s / n_vox * g.nvox
+    return {
+        "value": g.nrays / full_s / 1e9,
+        "unit": "GRays/s",
+        "cores": nt,
+        "kind": "port",
+        "sample": (f"oracle FP on {rays_done} rays ({n_rows} rows in each of {g.nviews} views) "
+                   f"+ BP on {n_vox} random voxels, {fp_s + bp_s:.1f} s, extrapolated to one "
+                   f"full forward+back pass ({full_s:.0f} s)"),
+        "seconds_per_step_extrapolated": full_s,
+    }
+
+
+def traffic_from_profiles(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["full", "small"], default="full")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "is a design in DESIGN.md, not a measured path", flush=True)
+        sys.exit(2)
+
+    import torch
+
+    import kevernals_amd as K
+    from kevernals_amd import geometry as G
+
+    g = G.full_plain() if args.workload == "full" else G.small_plain()
+    ctx = K.Context(0, g)
+    ctx.use_torch_stream()
+    mu, data = make_data(K, ctx, g, torch)
+    fp_pairs, bp_pairs = ctx.count_pairs(K.MODE_R)
+    log(f"[bench] accepted pairs: forward {fp_pairs}, backprojector {bp_pairs}")
+
+    x = torch.ones(g.vol_shape, device="cuda")
+    for _ in range(args.warmup):
+        ctx.mlem_forward(x, data, 1.0, K.MODE_R)
+        ctx.mlem_backward(x, K.MODE_R)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in ev:
+        e[0].record()
+        ctx.mlem_forward(x, data, 1.0, K.MODE_R)
+        e[1].record()
+        ctx.mlem_backward(x, K.MODE_R)
+        e[2].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    fp_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    bp_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    log(f"[bench] step {ms_step:.3f} ms (forward {fp_ms:.3f} ms, backward {bp_ms:.3f} ms)")
+
+    nr, nv = g.nrays, g.nvox
+    bytes_fp = 4 * fp_pairs + 8 * nr  # voxel gathers + data read + ratio write
+    bytes_bp = 4 * bp_pairs + 8 * nv  # ratio gathers + volume read + write
+    if bp_ms >= fp_ms:
+        kname, kms, kbytes = "bp_r_kernel<MLEM>", bp_ms, bytes_bp
+    else:
+        kname, kms, kbytes = "fp_r_kernel<MLEM>", fp_ms, bytes_fp
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    value = nr / (ms_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu = cpu_baseline(g, mu.cpu().numpy(), data.cpu().numpy(), args.cpu_seconds)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} GRays/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GRays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom + 200 microcalcifications, Poisson I0=5e3)",
+        "config": {
+            "workload": ("plain MLEM iteration, reference discretisation, fused forward+ratio / "
+                         "backprojection+update" if args.workload == "full" else "small plain MLEM iteration"),
+            "volume": [g.nx, g.ny, g.nz],
+            "voxel_mm": [g.sx, g.sy, g.sz],
+            "detector": [g.nu, g.nv],
+            "pixel_mm": [g.pu, g.pv],
+            "views": g.nviews,
+            "arc_deg": 20.0 if args.workload == "full" else 15.0,
+            "parallelism": "single GPU",
+        },
+        "mlem_iters_per_s": round(1e3 / ms_step, 3),
+        "kernels_ms": {"fp_r_kernel<MLEM>": round(fp_ms, 4), "bp_r_kernel<MLEM>": round(bp_ms, 4)},
+        "accepted_pairs": {"forward": fp_pairs, "backprojector": bp_pairs},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(kname),
+            "algorithmic_bytes": kbytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,155 @@
+/*
+ * kevernals_amd -- C ABI of the MI355X (gfx950) tomosynthesis projection and
+ * reconstruction path.
+ *
+ * Every entry point takes plain pointers and sizes and returns a kv_status;
+ * none of them exits the process.  Device-pointer entry points enqueue work on
+ * the context's HIP stream and return without synchronising; the *_host
+ * variants copy in, run and copy out synchronously (the calling convention of
+ * the reference's Projector, whose wrappers were synchronous).
+ *
+ * Layouts (as KevLoq/Kevernals source/modules/reconstruction/Projector.cu):
+ *   volume       float[nz][ny][nx], x fastest         (Projector.cu:361-363,401)
+ *   projections  float[np][nv][nu], u fastest         (Projector.cu:264,649,658)
+ *   world frame  millimetres, volume centred on the origin (Projector.cu:229-230)
+ *
+ * Operator modes:
+ *   KV_MODE_R  reference discretisation: weighted-mean ray-driven forward
+ *              projector (Projector.cu:227-372) and weighted-mean voxel-driven
+ *              +-5 pixel backprojector (Projector.cu:376-785).
+ *   KV_MODE_A  matched pair: Siddon line integrals in mm and their exact
+ *              transpose (no reference counterpart; used by the likelihood
+ *              and the learned steps).
+ */
+#ifndef KEVERNALS_AMD_H
+#define KEVERNALS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kv_status {
+    KV_OK = 0,
+    KV_ERR_INVALID = 1,      /* bad argument or geometry */
+    KV_ERR_DEVICE = 2,       /* HIP runtime failure */
+    KV_ERR_NOMEM = 3,        /* device allocation failed */
+    KV_ERR_STATE = 4,        /* call order (e.g. no geometry set) */
+    KV_ERR_PRECONDITION = 5  /* algorithmic precondition (ShiftAndAdd checks) */
+} kv_status;
+
+typedef enum kv_mode { KV_MODE_R = 0, KV_MODE_A = 1 } kv_mode;
+
+/* The kernel argument list of CudaPerformProjection /
+ * CudaPerformBackProjection (Projector.cu:227-236, 376-385) with per-view
+ * arrays as host pointers: det[3p+0..2] = detector bottom-left (x, y) and
+ * plane z of view p (Projector.cu:917-923); src[3p+0..2] = source position of
+ * view p (Projector.cu:930-936).  The reference's unused volume origin
+ * argument is dropped. */
+typedef struct kv_geometry {
+    int32_t nx, ny, nz;  /* volume size in voxels */
+    float sx, sy, sz;    /* voxel spacing, mm */
+    int32_t nu, nv, np;  /* detector pixels per row, rows, views */
+    float pu, pv;        /* pixel pitch, mm */
+    const float *det;    /* host, np*3 floats */
+    const float *src;    /* host, np*3 floats */
+} kv_geometry;
+
+typedef struct kv_context kv_context;
+
+/* ---- context ----------------------------------------------------------- */
+/* Replaces the per-call cudaMalloc/cudaMemcpy/launch/sync/free sequence of
+ * Projector::PerformProjection (Projector.cu:893-986): a context owns a
+ * stream, the device copy of the geometry and its precomputed tables. */
+kv_status kv_create(int device, kv_context **out);
+kv_status kv_destroy(kv_context *ctx);
+/* Use an external HIP stream (hipStream_t) for all later work; NULL restores
+ * the context's own stream. */
+kv_status kv_set_stream(kv_context *ctx, void *hip_stream);
+void *kv_get_stream(kv_context *ctx);
+kv_status kv_synchronize(kv_context *ctx);
+/* Message of the last failure on the calling thread ("" if none). */
+const char *kv_last_error(void);
+/* Library version string. */
+const char *kv_version(void);
+
+/* Geometry -> kernel arguments (Projector.cu:876-940).  Copies the per-view
+ * arrays to the device and builds the per-view crossing tables. */
+kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom);
+
+/* ---- device memory helpers (for hosts without their own allocator) ----- */
+kv_status kv_malloc(kv_context *ctx, size_t bytes, void **d_ptr);
+kv_status kv_free(kv_context *ctx, void *d_ptr);
+kv_status kv_memcpy_h2d(kv_context *ctx, void *d_dst, const void *h_src, size_t bytes);
+kv_status kv_memcpy_d2h(kv_context *ctx, void *h_dst, const void *d_src, size_t bytes);
+kv_status kv_memset(kv_context *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- operators (device pointers, asynchronous) --------------------------- */
+/* Forward projection: CudaPerformProjection (Projector.cu:227-372) for
+ * KV_MODE_R; line integrals in mm for KV_MODE_A. */
+kv_status kv_forward_project(kv_context *ctx, const float *d_vol, float *d_proj, int mode);
+/* Backprojection: CudaPerformBackProjection (Projector.cu:376-785) for
+ * KV_MODE_R; the exact transpose of the mode-A forward projector for
+ * KV_MODE_A. */
+kv_status kv_back_project(kv_context *ctx, const float *d_proj, float *d_vol, int mode);
+
+/* One iteration of recons::MLEM (Reconstructors.h:182-242), fused:
+ * the ratio relax*(data/FP) (Reconstructors.h:207-213) is the forward
+ * projector's epilogue and the multiplicative update (:238-241) the
+ * backprojector's.  d_vol is updated in place.  KV_MODE_A runs the standard
+ * x <- x/s * A^T(data/Ax) on line-integral data (relax ignored). */
+kv_status kv_mlem_iteration(kv_context *ctx, float *d_vol, const float *d_data, float relax, int mode);
+/* The two fused halves of kv_mlem_iteration, for callers that interleave
+ * work or time the passes separately: kv_mlem_forward writes the ratio
+ * (Reconstructors.h:185,207-213) into the context's work buffer,
+ * kv_mlem_backward backprojects it and applies the update
+ * (Reconstructors.h:225,238-241). */
+kv_status kv_mlem_forward(kv_context *ctx, const float *d_vol, const float *d_data, float relax, int mode);
+kv_status kv_mlem_backward(kv_context *ctx, float *d_vol, int mode);
+/* One iteration of recons::ART (Reconstructors.h:65-122), which is a
+ * simultaneous (SART-style) update: relax*(data - FP) fused into the forward
+ * projector, x += BP(...) into the backprojector.  Mode R only. */
+kv_status kv_sart_iteration(kv_context *ctx, float *d_vol, const float *d_data, float relax);
+
+/* recons::ShiftAndAdd (Reconstructors.h:263-379).  src_y (host, np floats)
+ * are the source y positions, plane_z (host, nplanes floats) the slice
+ * heights, sid the source-to-imager distance, fulcrum_z the fulcrum height,
+ * voxel_dy the volume y spacing.  d_out is float[nplanes][nv][nu].
+ * Returns KV_ERR_PRECONDITION for the reference's error exits
+ * (Reconstructors.h:275-285, 301-305, 319-323). */
+kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
+                           const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out);
+
+/* Poisson log-likelihood sum over rays with Ax > 0 of data*log(Ax) - Ax,
+ * Ax the mode-A forward projection of d_vol; reduced deterministically in
+ * FP64.  Synchronous (writes *out on return). */
+kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d_vol, double *out);
+
+/* Mode-A sensitivity s = A^T 1 into d_vol. */
+kv_status kv_sensitivity(kv_context *ctx, float *d_vol);
+
+/* ---- inspection (host pointers, synchronous) ----------------------------- */
+/* Walk sampled rays with the production kernels' walk and record the
+ * accepted (voxel index, weight) pairs: rays[i] = p*nu*nv + v*nu + u.
+ * idx/w hold nrays*cap entries, counts[i] the number of accepted pairs
+ * (may exceed cap; only cap are stored).  Mode R records the segment
+ * alpha-weights of Projector.cu:356-366, mode A the alpha-lengths. */
+kv_status kv_trace_rays(kv_context *ctx, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx,
+                        float *w, int32_t *counts);
+
+/* Number of accepted (ray, voxel) pairs of the forward walk (segments that
+ * pass Projector.cu:358-359) and of the backprojector (window pixels that
+ * pass Projector.cu:710-725); for mode A the pairs with positive length.
+ * These give the algorithmic bytes of a pass.  Synchronous. */
+kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *bp_pairs);
+
+/* ---- host-buffer variants (synchronous; used by the C++ Projector) ------- */
+kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode);
+kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""Python binding of libkevernals_amd (the C ABI in include/kevernals_amd.h).
+
+This is the host-side mirror used by tests and benchmarks; the product is the
+HIP library.  Device buffers are passed as torch tensors (device memory and
+streams are plumbing here) or raw device addresses.  There is no CPU
+fallback: without the built library, or without a GPU, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from .geometry import Geometry  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkevernals_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "kevernals_amd.h")
+
+MODE_R = 0
+MODE_A = 1
+
+KV_OK = 0
+STATUS_NAMES = {
+    0: "KV_OK",
+    1: "KV_ERR_INVALID",
+    2: "KV_ERR_DEVICE",
+    3: "KV_ERR_NOMEM",
+    4: "KV_ERR_STATE",
+    5: "KV_ERR_PRECONDITION",
+}
+
+
+class KvError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class _KvGeometry(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("sx", ctypes.c_float), ("sy", ctypes.c_float), ("sz", ctypes.c_float),
+        ("nu", ctypes.c_int32), ("nv", ctypes.c_int32), ("np", ctypes.c_int32),
+        ("pu", ctypes.c_float), ("pv", ctypes.c_float),
+        ("det", ctypes.POINTER(ctypes.c_float)), ("src", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SIGS = {
+    "kv_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "kv_destroy": (_I, [_P]),
+    "kv_set_stream": (_I, [_P, _P]),
+    "kv_get_stream": (_P, [_P]),
+    "kv_synchronize": (_I, [_P]),
+    "kv_last_error": (ctypes.c_char_p, []),
+    "kv_version": (ctypes.c_char_p, []),
+    "kv_set_geometry": (_I, [_P, ctypes.POINTER(_KvGeometry)]),
+    "kv_malloc": (_I, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "kv_free": (_I, [_P, _P]),
+    "kv_memcpy_h2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "kv_memcpy_d2h": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "kv_memset": (_I, [_P, _P, _I, ctypes.c_size_t]),
+    "kv_forward_project": (_I, [_P, _P, _P, _I]),
+    "kv_back_project": (_I, [_P, _P, _P, _I]),
+    "kv_mlem_iteration": (_I, [_P, _P, _P, _F, _I]),
+    "kv_mlem_forward": (_I, [_P, _P, _P, _F, _I]),
+    "kv_mlem_backward": (_I, [_P, _P, _I]),
+    "kv_count_pairs": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "kv_sart_iteration": (_I, [_P, _P, _P, _F]),
+    "kv_shift_and_add": (_I, [_P, _P, _P, _I, _P, _F, _F, _F, _P]),
+    "kv_log_likelihood": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+    "kv_sensitivity": (_I, [_P, _P]),
+    "kv_trace_rays": (_I, [_P, _I, _I, _P, _I, _P, _P, _P]),
+    "kv_forward_project_host": (_I, [_P, _P, _P, _I]),
+    "kv_back_project_host": (_I, [_P, _P, _P, _I]),
+}
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load libkevernals_amd.so (built by __graft_entry__.build()).  Raises
+    if it is missing: there is no fallback implementation."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libkevernals_amd.so not built at {p}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def declared_symbols(header: str = HEADER_PATH):
+    """Entry points declared in the public header."""
+    import re
+
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:kv_status|void|const char)\s*\*?\s*(kv_\w+)\s*\(", text, re.M)))
+
+
+def _ptr(x) -> int:
+    """Device address of a torch tensor, or an int address."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return x.data_ptr()
+    raise TypeError(f"cannot take a device address of {type(x)}")
+
+
+def _host_ptr(a: np.ndarray) -> int:
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+class Context:
+    """One kv_context: a device, a stream, a geometry and its tables."""
+
+    def __init__(self, device: int = 0, geometry: Optional[Geometry] = None):
+        self.lib = load_library()
+        h = _P()
+        self._check(self.lib.kv_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.geometry = None
+        if geometry is not None:
+            self.set_geometry(geometry)
+
+    def _check(self, status: int):
+        if status != KV_OK:
+            raise KvError(status, self.lib.kv_last_error().decode())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.kv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- plumbing ---------------------------------------------------------
+    def set_stream(self, stream_handle: Optional[int]):
+        self._check(self.lib.kv_set_stream(self.handle, stream_handle or 0))
+
+    def use_torch_stream(self):
+        """Enqueue on torch's current stream (so torch events time our work)."""
+        import torch
+
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def synchronize(self):
+        self._check(self.lib.kv_synchronize(self.handle))
+
+    def set_geometry(self, g: Geometry):
+        det = np.ascontiguousarray(g.det, dtype=np.float32)
+        src = np.ascontiguousarray(g.src, dtype=np.float32)
+        kg = _KvGeometry(g.nx, g.ny, g.nz, g.sx, g.sy, g.sz, g.nu, g.nv, g.nviews, g.pu, g.pv,
+                         det.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                         src.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        self._check(self.lib.kv_set_geometry(self.handle, ctypes.byref(kg)))
+        self.geometry = g
+
+    # -- operators (device buffers) ---------------------------------------
+    def forward_project(self, vol, proj, mode: int = MODE_R):
+        self._check(self.lib.kv_forward_project(self.handle, _ptr(vol), _ptr(proj), mode))
+
+    def back_project(self, proj, vol, mode: int = MODE_R):
+        self._check(self.lib.kv_back_project(self.handle, _ptr(proj), _ptr(vol), mode))
+
+    def mlem_iteration(self, vol, data, relax: float = 1.0, mode: int = MODE_R):
+        self._check(self.lib.kv_mlem_iteration(self.handle, _ptr(vol), _ptr(data), relax, mode))
+
+    def mlem_forward(self, vol, data, relax: float = 1.0, mode: int = MODE_R):
+        self._check(self.lib.kv_mlem_forward(self.handle, _ptr(vol), _ptr(data), relax, mode))
+
+    def mlem_backward(self, vol, mode: int = MODE_R):
+        self._check(self.lib.kv_mlem_backward(self.handle, _ptr(vol), mode))
+
+    def count_pairs(self, mode: int = MODE_R):
+        """(forward pairs, backprojector pairs) accepted by the two passes."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.kv_count_pairs(self.handle, mode, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def sart_iteration(self, vol, data, relax: float = 0.5):
+        self._check(self.lib.kv_sart_iteration(self.handle, _ptr(vol), _ptr(data), relax))
+
+    def sensitivity(self, vol):
+        self._check(self.lib.kv_sensitivity(self.handle, _ptr(vol)))
+
+    def shift_and_add(self, proj, out, src_y, plane_z, sid: float, fulcrum_z: float, voxel_dy: float):
+        sy = np.ascontiguousarray(src_y, dtype=np.float32)
+        pz = np.ascontiguousarray(plane_z, dtype=np.float32)
+        self._check(self.lib.kv_shift_and_add(self.handle, _ptr(proj), _host_ptr(sy), len(pz), _host_ptr(pz),
+                                              sid, fulcrum_z, voxel_dy, _ptr(out)))
+
+    def log_likelihood(self, data, vol) -> float:
+        out = ctypes.c_double()
+        self._check(self.lib.kv_log_likelihood(self.handle, _ptr(data), _ptr(vol), ctypes.byref(out)))
+        return out.value
+
+    def trace_rays(self, rays, cap: int = 1024, mode: int = MODE_R):
+        rays = np.ascontiguousarray(rays, dtype=np.int32)
+        n = len(rays)
+        idx = np.zeros((n, cap), dtype=np.int64)
+        w = np.zeros((n, cap), dtype=np.float32)
+        counts = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.kv_trace_rays(self.handle, mode, n, _host_ptr(rays), cap, _host_ptr(idx),
+                                           _host_ptr(w), _host_ptr(counts)))
+        return [(idx[i, : min(counts[i], cap)], w[i, : min(counts[i], cap)], int(counts[i])) for i in range(n)]
+
+    # -- host-buffer variants --------------------------------------------
+    def forward_project_host(self, vol: np.ndarray, mode: int = MODE_R) -> np.ndarray:
+        g = self.geometry
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        out = np.empty(g.proj_shape, dtype=np.float32)
+        self._check(self.lib.kv_forward_project_host(self.handle, _host_ptr(vol), _host_ptr(out), mode))
+        return out
+
+    def back_project_host(self, proj: np.ndarray, mode: int = MODE_R) -> np.ndarray:
+        g = self.geometry
+        proj = np.ascontiguousarray(proj, dtype=np.float32)
+        out = np.empty(g.vol_shape, dtype=np.float32)
+        self._check(self.lib.kv_back_project_host(self.handle, _host_ptr(proj), _host_ptr(out), mode))
+        return out

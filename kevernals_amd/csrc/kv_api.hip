@@ -1,0 +1,690 @@
+// C ABI of libkevernals_amd: context, geometry marshalling and dispatch.
+//
+// Replaces the host half of KevLoq/Kevernals Projector.cu (:873-1207), which
+// allocated, copied, launched, synchronised and freed on every call and
+// exited the process on CUDA errors (CudaErrorHandler.h).  Here a context
+// keeps the geometry, its crossing tables and work buffers resident, work is
+// enqueued on one stream, and every failure comes back as a kv_status with a
+// message (kv_last_error).
+#pragma clang fp contract(off)
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kv_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class T>
+void free_dev(T *&p)
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_tables(KvTables &t)
+{
+    free_dev(t.x);
+    free_dev(t.y);
+    free_dev(t.z);
+    free_dev(t.lx2);
+    free_dev(t.ly2);
+    free_dev(t.lz2);
+}
+
+kv_status dev_alloc(void **p, size_t bytes)
+{
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        kv_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? KV_ERR_NOMEM : KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+kv_status check_ctx(kv_context *ctx, bool need_geom)
+{
+    if (!ctx) {
+        kv_set_error("null context");
+        return KV_ERR_INVALID;
+    }
+    if (need_geom && !ctx->has_geom) {
+        kv_set_error("no geometry set on this context (call kv_set_geometry first)");
+        return KV_ERR_STATE;
+    }
+    const hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) {
+        kv_set_error("hipSetDevice(%d) failed: %s", ctx->device, hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+kv_status check_mode(int mode)
+{
+    if (mode != KV_MODE_R && mode != KV_MODE_A) {
+        kv_set_error("unknown operator mode %d", mode);
+        return KV_ERR_INVALID;
+    }
+    return KV_OK;
+}
+
+kv_status ensure_tables(kv_context *ctx, int mode)
+{
+    const KvGeom &g = ctx->g;
+    const size_t nxe = (size_t)g.np * g.nu * (g.nx + 1), nye = (size_t)g.np * g.nv * (g.ny + 1),
+                 nze = (size_t)g.np * (g.nz + 1);
+    if (mode == KV_MODE_R) {
+        if (ctx->tab_r_ok)
+            return KV_OK;
+        KvTables &t = ctx->tab_r;
+        kv_status st;
+        if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
+            (st = dev_alloc((void **)&t.z, nze * 4))) {
+            free_tables(t);
+            return st;
+        }
+        KV_TRY_HIP(kv_launch_tables_r(g, t, ctx->stream));
+        ctx->tab_r_ok = true;
+        return KV_OK;
+    }
+    if (ctx->tab_a_ok)
+        return KV_OK;
+    KvTables &t = ctx->tab_a;
+    kv_status st;
+    if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
+        (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.lx2, (size_t)g.np * g.nu * 4)) ||
+        (st = dev_alloc((void **)&t.ly2, (size_t)g.np * g.nv * 4)) ||
+        (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4))) {
+        free_tables(t);
+        return st;
+    }
+    KV_TRY_HIP(kv_launch_tables_a(g, t, ctx->stream));
+    ctx->tab_a_ok = true;
+    return KV_OK;
+}
+
+kv_status ensure_sens(kv_context *ctx)
+{
+    if (ctx->sens_ok)
+        return KV_OK;
+    kv_status st = ensure_tables(ctx, KV_MODE_A);
+    if (st)
+        return st;
+    if (!ctx->d_sens && (st = dev_alloc((void **)&ctx->d_sens, (size_t)kv_nvox(ctx->g) * 4)))
+        return st;
+    KV_TRY_HIP(kv_launch_bp_a(ctx->g, ctx->tab_a, nullptr, ctx->d_sens, KV_EP_PLAIN, nullptr, true, ctx->stream));
+    ctx->sens_ok = true;
+    return KV_OK;
+}
+
+void release_geometry(kv_context *ctx)
+{
+    free_dev(ctx->d_det);
+    free_dev(ctx->d_src);
+    free_tables(ctx->tab_r);
+    free_tables(ctx->tab_a);
+    free_dev(ctx->d_ratio);
+    free_dev(ctx->d_sens);
+    ctx->tab_r_ok = ctx->tab_a_ok = ctx->sens_ok = false;
+    ctx->has_geom = false;
+}
+
+bool finite_pos(float v) { return std::isfinite(v) && v > 0.f; }
+
+// AlmostEqualRelative of commons/Maths.h:25-34 (FP32, epsilon default).
+bool almost_equal_relative(float a, float b)
+{
+    const float diff = std::fabs(a - b);
+    a = std::fabs(a);
+    b = std::fabs(b);
+    const float m = (a < b) ? b : a;
+    return diff <= FLT_EPSILON * m;
+}
+}  // namespace
+
+void kv_set_error(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+extern "C" {
+
+const char *kv_last_error(void) { return g_last_error.c_str(); }
+
+const char *kv_version(void) { return "kevernals_amd 0.1 (gfx950)"; }
+
+kv_status kv_create(int device, kv_context **out)
+{
+    if (!out) {
+        kv_set_error("null output pointer");
+        return KV_ERR_INVALID;
+    }
+    *out = nullptr;
+    int count = 0;
+    KV_TRY_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) {
+        kv_set_error("device %d out of range (%d devices)", device, count);
+        return KV_ERR_INVALID;
+    }
+    KV_TRY_HIP(hipSetDevice(device));
+    auto *ctx = new kv_context();
+    ctx->device = device;
+    const hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        kv_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+        delete ctx;
+        return KV_ERR_DEVICE;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return KV_OK;
+}
+
+kv_status kv_destroy(kv_context *ctx)
+{
+    if (!ctx)
+        return KV_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_geometry(ctx);
+    free_dev(ctx->d_red);
+    if (ctx->own_stream)
+        (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return KV_OK;
+}
+
+kv_status kv_set_stream(kv_context *ctx, void *hip_stream)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    return KV_OK;
+}
+
+void *kv_get_stream(kv_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+kv_status kv_synchronize(kv_context *ctx)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    if (!geom || !geom->det || !geom->src) {
+        kv_set_error("null geometry or per-view arrays");
+        return KV_ERR_INVALID;
+    }
+    const kv_geometry &q = *geom;
+    if (q.nx <= 0 || q.ny <= 0 || q.nz <= 0 || q.nu <= 0 || q.nv <= 0 || q.np <= 0) {
+        kv_set_error("non-positive size (volume %d x %d x %d, detector %d x %d x %d)", q.nx, q.ny, q.nz, q.nu, q.nv,
+                     q.np);
+        return KV_ERR_INVALID;
+    }
+    if (!finite_pos(q.sx) || !finite_pos(q.sy) || !finite_pos(q.sz) || !finite_pos(q.pu) || !finite_pos(q.pv)) {
+        kv_set_error("voxel spacing and pixel pitch must be finite and positive");
+        return KV_ERR_INVALID;
+    }
+    // voxel indices are 32-bit unsigned in the kernels; ray ids 31-bit
+    const int64_t nvox = (int64_t)q.nx * q.ny * q.nz, nray = (int64_t)q.np * q.nu * q.nv;
+    if (nvox > 0xFFFFFFFFLL || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24)) {
+        kv_set_error("problem too large: %lld voxels, %lld rays", (long long)nvox, (long long)nray);
+        return KV_ERR_INVALID;
+    }
+    for (int i = 0; i < 3 * q.np; ++i)
+        if (!std::isfinite(q.det[i]) || !std::isfinite(q.src[i])) {
+            kv_set_error("non-finite source or detector coordinate at index %d", i);
+            return KV_ERR_INVALID;
+        }
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    release_geometry(ctx);
+    kv_status st;
+    if ((st = dev_alloc((void **)&ctx->d_det, (size_t)q.np * 12)) ||
+        (st = dev_alloc((void **)&ctx->d_src, (size_t)q.np * 12)) ||
+        (st = dev_alloc((void **)&ctx->d_ratio, (size_t)nray * 4))) {
+        release_geometry(ctx);
+        return st;
+    }
+    KV_TRY_HIP(hipMemcpy(ctx->d_det, q.det, (size_t)q.np * 12, hipMemcpyHostToDevice));
+    KV_TRY_HIP(hipMemcpy(ctx->d_src, q.src, (size_t)q.np * 12, hipMemcpyHostToDevice));
+    KvGeom &g = ctx->g;
+    g.nx = q.nx; g.ny = q.ny; g.nz = q.nz;
+    g.nu = q.nu; g.nv = q.nv; g.np = q.np;
+    g.sx = q.sx; g.sy = q.sy; g.sz = q.sz;
+    g.pu = q.pu; g.pv = q.pv;
+    g.det = ctx->d_det;
+    g.src = ctx->d_src;
+    ctx->has_geom = true;
+    return KV_OK;
+}
+
+kv_status kv_malloc(kv_context *ctx, size_t bytes, void **d_ptr)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    if (!d_ptr) {
+        kv_set_error("null output pointer");
+        return KV_ERR_INVALID;
+    }
+    return dev_alloc(d_ptr, bytes ? bytes : 1);
+}
+
+kv_status kv_free(kv_context *ctx, void *d_ptr)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    if (d_ptr)
+        KV_TRY_HIP(hipFree(d_ptr));
+    return KV_OK;
+}
+
+kv_status kv_memcpy_h2d(kv_context *ctx, void *d_dst, const void *h_src, size_t bytes)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    KV_TRY_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_memcpy_d2h(kv_context *ctx, void *h_dst, const void *d_src, size_t bytes)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    KV_TRY_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_memset(kv_context *ctx, void *d_dst, int value, size_t bytes)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    KV_TRY_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_forward_project(kv_context *ctx, const float *d_vol, float *d_proj, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (!d_vol || !d_proj) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    if (mode == KV_MODE_R)
+        KV_TRY_HIP(kv_launch_fp_r(ctx->g, d_vol, d_proj, KV_EP_PLAIN, nullptr, 0.f, ctx->stream));
+    else
+        KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, d_proj, KV_EP_PLAIN, nullptr, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_back_project(kv_context *ctx, const float *d_proj, float *d_vol, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (!d_vol || !d_proj) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = ensure_tables(ctx, mode))
+        return st;
+    if (mode == KV_MODE_R)
+        KV_TRY_HIP(kv_launch_bp_r(ctx->g, ctx->tab_r, d_proj, d_vol, KV_EP_PLAIN, ctx->stream));
+    else
+        KV_TRY_HIP(kv_launch_bp_a(ctx->g, ctx->tab_a, d_proj, d_vol, KV_EP_PLAIN, nullptr, false, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_mlem_forward(kv_context *ctx, const float *d_vol, const float *d_data, float relax, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (!d_vol || !d_data) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    // tables (and the mode-A sensitivity) are built here so that the
+    // backward half enqueues a single kernel
+    if (kv_status st = (mode == KV_MODE_R) ? ensure_tables(ctx, KV_MODE_R) : ensure_sens(ctx))
+        return st;
+    if (mode == KV_MODE_R)
+        KV_TRY_HIP(kv_launch_fp_r(ctx->g, d_vol, ctx->d_ratio, KV_EP_MLEM, d_data, relax, ctx->stream));
+    else
+        KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, ctx->d_ratio, KV_EP_MLEM, d_data, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_mlem_backward(kv_context *ctx, float *d_vol, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (!d_vol) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = (mode == KV_MODE_R) ? ensure_tables(ctx, KV_MODE_R) : ensure_sens(ctx))
+        return st;
+    if (mode == KV_MODE_R)
+        KV_TRY_HIP(kv_launch_bp_r(ctx->g, ctx->tab_r, ctx->d_ratio, d_vol, KV_EP_MLEM, ctx->stream));
+    else
+        KV_TRY_HIP(
+            kv_launch_bp_a(ctx->g, ctx->tab_a, ctx->d_ratio, d_vol, KV_EP_MLEM, ctx->d_sens, false, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_mlem_iteration(kv_context *ctx, float *d_vol, const float *d_data, float relax, int mode)
+{
+    if (kv_status st = kv_mlem_forward(ctx, d_vol, d_data, relax, mode))
+        return st;
+    return kv_mlem_backward(ctx, d_vol, mode);
+}
+
+kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *bp_pairs)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (!fp_pairs || !bp_pairs) {
+        kv_set_error("null output pointer");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = ensure_tables(ctx, mode))
+        return st;
+    const int nb = 1024;
+    if (ctx->red_cap < nb) {
+        free_dev(ctx->d_red);
+        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
+            return st;
+        ctx->red_cap = nb;
+    }
+    float *d_cnt = nullptr;
+    if (kv_status st = dev_alloc((void **)&d_cnt, (size_t)kv_nvox(ctx->g) * 4))
+        return st;
+    std::vector<double> part(nb);
+    double sums[2] = {0.0, 0.0};
+    hipError_t e = hipSuccess;
+    for (int pass = 0; pass < 2 && e == hipSuccess; ++pass) {
+        float *buf = pass == 0 ? ctx->d_ratio : d_cnt;
+        const int64_t n = pass == 0 ? kv_nray(ctx->g) : kv_nvox(ctx->g);
+        if (pass == 0)
+            e = mode == KV_MODE_R ? kv_launch_fp_r(ctx->g, nullptr, buf, KV_EP_COUNT, nullptr, 0.f, ctx->stream)
+                                  : kv_launch_fp_a(ctx->g, nullptr, buf, KV_EP_COUNT, nullptr, ctx->stream);
+        else
+            e = mode == KV_MODE_R ? kv_launch_bp_r(ctx->g, ctx->tab_r, nullptr, buf, KV_EP_COUNT, ctx->stream)
+                                  : kv_launch_bp_a(ctx->g, ctx->tab_a, nullptr, buf, KV_EP_COUNT, nullptr, true,
+                                                   ctx->stream);
+        if (e == hipSuccess)
+            e = kv_launch_sum(n, buf, ctx->d_red, nb, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(part.data(), ctx->d_red, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        for (int i = 0; i < nb && e == hipSuccess; ++i)
+            sums[pass] += part[i];
+    }
+    free_dev(d_cnt);
+    if (e != hipSuccess) {
+        kv_set_error("pair count failed: %s", hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    *fp_pairs = (int64_t)sums[0];
+    *bp_pairs = (int64_t)sums[1];
+    return KV_OK;
+}
+
+kv_status kv_sart_iteration(kv_context *ctx, float *d_vol, const float *d_data, float relax)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_vol || !d_data) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = ensure_tables(ctx, KV_MODE_R))
+        return st;
+    KV_TRY_HIP(kv_launch_fp_r(ctx->g, d_vol, ctx->d_ratio, KV_EP_SART, d_data, relax, ctx->stream));
+    KV_TRY_HIP(kv_launch_bp_r(ctx->g, ctx->tab_r, ctx->d_ratio, d_vol, KV_EP_SART, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_sensitivity(kv_context *ctx, float *d_vol)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_vol) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = ensure_tables(ctx, KV_MODE_A))
+        return st;
+    KV_TRY_HIP(kv_launch_bp_a(ctx->g, ctx->tab_a, nullptr, d_vol, KV_EP_PLAIN, nullptr, true, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
+                           const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_proj || !d_out || !src_y || (nplanes > 0 && !plane_z)) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    const KvGeom &g = ctx->g;
+    // Reconstructors.h:274-305
+    if (nplanes <= 1) {
+        kv_set_error("ShiftAndAdd: required nb of slices in reconstruction %d <= 1", nplanes);
+        return KV_ERR_PRECONDITION;
+    }
+    if (g.np <= 1) {
+        kv_set_error("ShiftAndAdd: required nb of projections %d <= 1", g.np);
+        return KV_ERR_PRECONDITION;
+    }
+    if (almost_equal_relative(fulcrum_z, sid)) {
+        kv_set_error("ShiftAndAdd: Fulcrum and imager are at the same height: impossible");
+        return KV_ERR_PRECONDITION;
+    }
+    if (!finite_pos(voxel_dy)) {
+        kv_set_error("ShiftAndAdd: voxel y spacing must be finite and positive");
+        return KV_ERR_INVALID;
+    }
+    // Reconstructors.h:308-331: per-plane integer row shifts
+    const float mF = sid / (sid - fulcrum_z);
+    std::vector<int32_t> shifts((size_t)nplanes * g.np);
+    for (int k = 0; k < nplanes; ++k) {
+        const float z = plane_z[k];
+        if (almost_equal_relative(z, sid)) {
+            kv_set_error("ShiftAndAdd: reconstruction slice and imager are at the same height: impossible");
+            return KV_ERR_PRECONDITION;
+        }
+        const float mZ = sid / (sid - z);
+        const float coeff = mZ - mF;
+        for (int p = 0; p < g.np; ++p) {
+            const float world = src_y[p] * coeff;
+            const float t = world / voxel_dy;
+            if (!(std::fabs(t) < 2.0e9f)) {
+                kv_set_error("ShiftAndAdd: shift of plane %d view %d out of range", k, p);
+                return KV_ERR_INVALID;
+            }
+            shifts[(size_t)k * g.np + p] = (int32_t)t;
+        }
+    }
+    int32_t *d_shifts = nullptr;
+    if (kv_status st = dev_alloc((void **)&d_shifts, shifts.size() * 4))
+        return st;
+    hipError_t e = hipMemcpyAsync(d_shifts, shifts.data(), shifts.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = kv_launch_saa(g.nu, g.nv, g.np, d_proj, d_shifts, nplanes, d_out, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_shifts);
+    if (e != hipSuccess) {
+        kv_set_error("shift-and-add failed: %s", hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d_vol, double *out)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_data || !d_vol || !out) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    const int nb = 1024;
+    if (ctx->red_cap < nb) {
+        free_dev(ctx->d_red);
+        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
+            return st;
+        ctx->red_cap = nb;
+    }
+    KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, ctx->d_ratio, KV_EP_PLAIN, nullptr, ctx->stream));
+    KV_TRY_HIP(kv_launch_loglik(ctx->g, d_data, ctx->d_ratio, ctx->d_red, nb, ctx->stream));
+    std::vector<double> part(nb);
+    KV_TRY_HIP(hipMemcpyAsync(part.data(), ctx->d_red, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (int i = 0; i < nb; ++i)
+        s += part[i];
+    *out = s;
+    return KV_OK;
+}
+
+kv_status kv_trace_rays(kv_context *ctx, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx, float *w,
+                        int32_t *counts)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (kv_status st = check_mode(mode))
+        return st;
+    if (nrays <= 0 || cap <= 0 || !rays || !idx || !w || !counts) {
+        kv_set_error("bad trace arguments");
+        return KV_ERR_INVALID;
+    }
+    const int64_t nr = kv_nray(ctx->g);
+    for (int i = 0; i < nrays; ++i)
+        if (rays[i] < 0 || rays[i] >= nr) {
+            kv_set_error("ray id %d out of range", rays[i]);
+            return KV_ERR_INVALID;
+        }
+    int32_t *d_rays = nullptr, *d_counts = nullptr;
+    int64_t *d_idx = nullptr;
+    float *d_w = nullptr;
+    kv_status st = KV_OK;
+    const size_t slots = (size_t)nrays * cap;
+    if ((st = dev_alloc((void **)&d_rays, (size_t)nrays * 4)) || (st = dev_alloc((void **)&d_counts, (size_t)nrays * 4)) ||
+        (st = dev_alloc((void **)&d_idx, slots * 8)) || (st = dev_alloc((void **)&d_w, slots * 4))) {
+        free_dev(d_rays);
+        free_dev(d_counts);
+        free_dev(d_idx);
+        free_dev(d_w);
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(d_rays, rays, (size_t)nrays * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = kv_launch_trace(ctx->g, mode, nrays, d_rays, cap, d_idx, d_w, d_counts, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(idx, d_idx, slots * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(w, d_w, slots * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts, d_counts, (size_t)nrays * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    free_dev(d_rays);
+    free_dev(d_counts);
+    free_dev(d_idx);
+    free_dev(d_w);
+    if (e != hipSuccess) {
+        kv_set_error("trace failed: %s", hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+static kv_status host_roundtrip(kv_context *ctx, const float *h_in, size_t in_n, float *h_out, size_t out_n, int mode,
+                                bool forward)
+{
+    float *d_in = nullptr, *d_out = nullptr;
+    kv_status st;
+    if ((st = dev_alloc((void **)&d_in, in_n * 4)) || (st = dev_alloc((void **)&d_out, out_n * 4))) {
+        free_dev(d_in);
+        free_dev(d_out);
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(d_in, h_in, in_n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        kv_set_error("copy to device failed: %s", hipGetErrorString(e));
+        st = KV_ERR_DEVICE;
+    } else {
+        st = forward ? kv_forward_project(ctx, d_in, d_out, mode) : kv_back_project(ctx, d_in, d_out, mode);
+    }
+    if (st == KV_OK) {
+        e = hipMemcpyAsync(h_out, d_out, out_n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            kv_set_error("copy to host failed: %s", hipGetErrorString(e));
+            st = KV_ERR_DEVICE;
+        }
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    free_dev(d_in);
+    free_dev(d_out);
+    return st;
+}
+
+kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!h_vol || !h_proj) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    return host_roundtrip(ctx, h_vol, (size_t)kv_nvox(ctx->g), h_proj, (size_t)kv_nray(ctx->g), mode, true);
+}
+
+kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!h_vol || !h_proj) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    return host_roundtrip(ctx, h_proj, (size_t)kv_nray(ctx->g), h_vol, (size_t)kv_nvox(ctx->g), mode, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,430 @@
+// Forward projectors for gfx950.
+//
+// One ray per lane; a 256-thread workgroup is four wavefronts, each covering
+// 64 consecutive detector pixels of one row, so the 64 rays of a wavefront are
+// neighbours in u and their voxel gathers at each step fall on neighbouring
+// x addresses of the x-fastest volume.
+//
+// Mode R reproduces CudaPerformProjection (KevLoq/Kevernals
+// source/modules/reconstruction/Projector.cu:227-372) bit for bit.  The
+// reference fills a per-thread 2000-float array with 1.0, writes three
+// arithmetic runs of plane crossings into it and selection-sorts it
+// (FindProjectionAlphas_d, Projector.cu:45-223).  Each run is non-decreasing
+// (a[i] = a[i-1] + |1/d| in FP32), so the sorted list is exactly the merge of
+// the x run, the y run, the (truncated) z run and the leftover 1.0 fillers:
+// here the four runs are merged on the fly in registers, with no scratch
+// array and O(n) work instead of O(n^2).
+//
+// All arithmetic that decides voxel indices is evaluated as the reference
+// writes it, with contraction off.
+#pragma clang fp contract(off)
+
+#include "kv_internal.h"
+
+namespace {
+
+constexpr float kInf = __builtin_inff();
+
+// Projector.cu:22-42
+__device__ __forceinline__ bool feq(float a, float b, float t)
+{
+    if (t < 0)
+        return (a - b) < -t && (a - b) > t;
+    return (a - b) < t && (a - b) > -t;
+}
+__device__ __forceinline__ bool fsupeq(float a, float b, float t) { return a > b || feq(a, b, t); }
+
+struct RayF {
+    float sx, sy, sz;  // source, floating-voxel units
+    float dx, dy, dz;  // source -> pixel centre
+    float hx, hy, hz;  // half sizes
+};
+
+// Projector.cu:303-323
+__device__ __forceinline__ void ray_setup(const KvGeom &g, int p, int u, int v, RayF &r)
+{
+    const float *S = g.src + 3 * p;
+    const float *D = g.det + 3 * p;
+    r.sx = S[0] / g.sx;
+    r.sy = S[1] / g.sy;
+    r.sz = S[2] / g.sz;
+    const float qx = (D[0] + ((float)u + 0.5f) * g.pu) / g.sx;
+    const float qy = (D[1] + ((float)v + 0.5f) * g.pv) / g.sy;
+    const float qz = D[2] / g.sz;
+    r.dx = qx - r.sx;
+    r.dy = qy - r.sy;
+    r.dz = qz - r.sz;
+    r.hx = (float)g.nx / 2.f;
+    r.hy = (float)g.ny / 2.f;
+    r.hz = (float)g.nz / 2.f;
+}
+
+// Projector.cu:88-120
+__device__ __forceinline__ void axis_range(float d, float s, float h, int n, float amin, float amax, int &lo, int &hi)
+{
+    int mx, mn;
+    if (d > 0) {
+        mx = max(0, min(n - 1, (int)(h + s + amax * d)));
+        mn = max(0, min(mx, n - (int)(h - s - amin * d)));
+    } else {
+        mx = max(0, min(n - 1, (int)(h + s + amin * d)));
+        mn = max(0, min(mx, n - (int)(h - s - amax * d)));
+    }
+    lo = mn;
+    hi = mx;
+}
+
+// Mode-R walk of one ray.  visit(voxel_index, weight) is called for every
+// accepted segment (Projector.cu:356-366) in the order of the sorted list.
+template <class Visit>
+__device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&visit)
+{
+    const float tol = 1e-6f;
+    const bool okx = fabsf(r.dx) > tol, oky = fabsf(r.dy) > tol, okz = fabsf(r.dz) > tol;
+    const float a1x = okx ? (-r.hx - r.sx) / r.dx : 0.f;
+    const float a1y = oky ? (-r.hy - r.sy) / r.dy : 0.f;
+    const float a1z = okz ? (-r.hz - r.sz) / r.dz : 0.f;
+    const float aNx = okx ? (r.hx - r.sx) / r.dx : 0.f;
+    const float aNy = oky ? (r.hy - r.sy) / r.dy : 0.f;
+    const float aNz = okz ? (r.hz - r.sz) / r.dz : 0.f;
+    const float amin = fmaxf(fmaxf(0.f, fminf(a1x, aNx)), fmaxf(fminf(a1y, aNy), fminf(a1z, aNz)));
+    const float amax = fminf(fminf(1.f, fmaxf(a1x, aNx)), fminf(fmaxf(a1y, aNy), fmaxf(a1z, aNz)));
+    if (fsupeq(amin, amax, 1e-7f))
+        return;  // the reference's early exit (Projector.cu:79-85): empty list
+
+    int lox, hix, loy, hiy, loz, hiz;
+    axis_range(r.dx, r.sx, r.hx, g.nx, amin, amax, lox, hix);
+    axis_range(r.dy, r.sy, r.hy, g.ny, amin, amax, loy, hiy);
+    axis_range(r.dz, r.sz, r.hz, g.nz, amin, amax, loz, hiz);
+    const int n = (hix - lox) + (hiy - loy) + (hiz - loz) + 3;
+
+    // Run heads, remaining counts and increments (Projector.cu:139-201).
+    // A run whose axis fails |d| > tol leaves its slots at the 1.0 filler.
+    int cx = 0, cy = 0, cz = 0;
+    float ax = 0.f, ay = 0.f, az = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
+    if (okx) {
+        const float st = 1.f / r.dx;
+        ax = st * (-r.hx - r.sx + (float)(r.dx > 0 ? lox : hix));
+        ix = fabsf(st);
+        cx = hix - lox + 1;
+    }
+    if (oky) {
+        const float st = 1.f / r.dy;
+        ay = st * (-r.hy - r.sy + (float)(r.dy > 0 ? loy : hiy));
+        iy = fabsf(st);
+        cy = hiy - loy + 1;
+    }
+    if (okz) {
+        // z run written over (hiz-loz)-1 slots, at least its first entry
+        // (Projector.cu:181-201).
+        const float st = 1.f / r.dz;
+        az = st * (-r.hz - r.sz + (float)(r.dz > 0 ? loz : hiz));
+        iz = fabsf(st);
+        const int dzr = hiz - loz;
+        cz = dzr >= 2 ? dzr - 1 : 1;
+    }
+    int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
+
+    auto pop = [&]() -> float {
+        const float vx = cx > 0 ? ax : kInf;
+        const float vy = cy > 0 ? ay : kInf;
+        const float vz = cz > 0 ? az : kInf;
+        const float vf = cf > 0 ? 1.0f : kInf;
+        const float m = fminf(fminf(vx, vy), fminf(vz, vf));
+        const bool tx = vx == m;
+        const bool ty = !tx && vy == m;
+        const bool tz = !tx && !ty && vz == m;
+        const bool tf = !tx && !ty && !tz;
+        ax = tx ? ax + ix : ax;
+        ay = ty ? ay + iy : ay;
+        az = tz ? az + iz : az;
+        cx -= tx;
+        cy -= ty;
+        cz -= tz;
+        cf -= tf;
+        return m;
+    };
+
+    // Segment loop, Projector.cu:348-367.  The midpoint is the FP32 sum
+    // divided by 2.0 in FP64; (double)float * (double)float is exact, so the
+    // index expression equals one FP64 fma.
+    const double bx = (double)(r.hx + r.sx);
+    const double by = (double)(r.hy + r.sy);
+    const double bz = (double)(r.hz + r.sz);
+    const double ddx = (double)r.dx, ddy = (double)r.dy, ddz = (double)r.dz;
+    const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz;
+    float prev = pop();
+    for (int i = 0; i < n - 1; ++i) {
+        const float cur = pop();
+        const float w = cur - prev;
+        const double mid = (double)(cur + prev) * 0.5;
+        const int vx = (int)__builtin_fma(mid, ddx, bx);
+        const int vy = (int)__builtin_fma(mid, ddy, by);
+        const int vz = (int)__builtin_fma(mid, ddz, bz);
+        if ((unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol)
+            visit(((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx, w);
+        prev = cur;
+    }
+}
+
+// AlmostEqualRelative(v, 0) of commons/Maths.h:25-34 in FP32.
+__device__ __forceinline__ bool almost_equal_zero(float a)
+{
+    const float diff = fabsf(a - 0.f);
+    const float A = fabsf(a), B = 0.f;
+    const float m = (A < B) ? B : A;  // std::max
+    return diff <= __FLT_EPSILON__ * m;
+}
+
+template <int EP>
+__global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
+                                                   const float *__restrict__ data, float relax)
+{
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (u >= g.nu || v >= g.nv)
+        return;
+    RayF r;
+    ray_setup(g, p, u, v, r);
+    const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
+    if (EP == KV_EP_COUNT) {
+        int c = 0;
+        walk_r(g, r, [&](unsigned, float) { ++c; });
+        out[o] = (float)c;
+        return;
+    }
+    float tot = 0.f, tw = 0.f;
+    walk_r(g, r, [&](unsigned k, float w) {
+        tw += w;
+        tot += w * vol[k];
+    });
+    const float fp = tw > 1e-6f ? tot / tw : 0.f;
+    if (EP == KV_EP_PLAIN) {
+        out[o] = fp;
+    } else if (EP == KV_EP_MLEM) {
+        // Reconstructors.h:207-213: the ratio overwrites the projection
+        // unless it is (almost) zero, where the projection value stays.
+        out[o] = almost_equal_zero(fp) ? fp : relax * (data[o] / fp);
+    } else {
+        // Reconstructors.h:90-93
+        out[o] = relax * (data[o] - fp);
+    }
+}
+
+// ---------------------------------------------------------------- mode A --
+struct RayA {
+    float s[3], d[3], inv[3], h[3];
+    int n[3];
+    int fixed[3];
+    float L;
+};
+
+__device__ __forceinline__ void ray_setup_a(const KvGeom &g, int p, int u, int v, RayA &r)
+{
+    RayF f;
+    ray_setup(g, p, u, v, f);
+    r.s[0] = f.sx; r.s[1] = f.sy; r.s[2] = f.sz;
+    r.d[0] = f.dx; r.d[1] = f.dy; r.d[2] = f.dz;
+    r.h[0] = f.hx; r.h[1] = f.hy; r.h[2] = f.hz;
+    r.n[0] = g.nx; r.n[1] = g.ny; r.n[2] = g.nz;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.inv[c] = r.d[c] != 0.f ? 1.f / r.d[c] : 0.f;
+        r.fixed[c] = 0;
+        if (r.d[c] == 0.f) {
+            const float pos = r.s[c] + r.h[c];
+            r.fixed[c] = (pos > 0.f && pos < (float)r.n[c]) ? (int)floorf(pos) : -1;
+        }
+    }
+    const float lx = r.d[0] * g.sx, ly = r.d[1] * g.sy, lz = r.d[2] * g.sz;
+    r.L = sqrtf(lx * lx + ly * ly + lz * lz);
+}
+
+__device__ __forceinline__ float plane_a(const RayA &r, int c, int k)
+{
+    return (((float)k - r.h[c]) - r.s[c]) * r.inv[c];
+}
+
+// Mode-A Siddon walk: visit(voxel, len) for every voxel with len > 0.
+template <class Visit>
+__device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
+{
+    float ain = 0.f, aout = 1.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (r.d[c] == 0.f) {
+            if (r.fixed[c] < 0)
+                return;
+            continue;
+        }
+        const float a0 = plane_a(r, c, 0), aN = plane_a(r, c, r.n[c]);
+        ain = fmaxf(ain, fminf(a0, aN));
+        aout = fminf(aout, fmaxf(a0, aN));
+    }
+    if (!(ain < aout))
+        return;
+    int ii[3];
+    float lo[3], hi[3];
+    int stp[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (r.d[c] == 0.f) {
+            ii[c] = r.fixed[c];
+            lo[c] = -kInf;
+            hi[c] = kInf;
+            stp[c] = 0;
+            continue;
+        }
+        const float pos = r.s[c] + r.h[c] + ain * r.d[c];
+        int i = (int)floorf(pos);
+        i = max(0, min(r.n[c] - 1, i));
+        const bool up = r.d[c] > 0.f;
+        stp[c] = up ? 1 : -1;
+        // lo/hi of voxel i on axis c
+        auto lohi = [&](int k, float &l, float &h) {
+            const float a0 = plane_a(r, c, k), a1 = plane_a(r, c, k + 1);
+            l = fminf(a0, a1);
+            h = fmaxf(a0, a1);
+        };
+        float l, h;
+        for (;;) {
+            lohi(i, l, h);
+            const int j = i - stp[c];
+            if (l > ain && j >= 0 && j < r.n[c]) i = j; else break;
+        }
+        for (;;) {
+            lohi(i, l, h);
+            const int j = i + stp[c];
+            if (h <= ain && j >= 0 && j < r.n[c]) i = j; else break;
+        }
+        ii[c] = i;
+        lo[c] = l;
+        hi[c] = h;
+    }
+    const int guard = r.n[0] + r.n[1] + r.n[2] + 3;
+    const unsigned nx = (unsigned)r.n[0], ny = (unsigned)r.n[1];
+    for (int step = 0; step < guard; ++step) {
+        const float top = fminf(fminf(1.f, hi[0]), fminf(hi[1], hi[2]));
+        const float bot = fmaxf(fmaxf(0.f, lo[0]), fmaxf(lo[1], lo[2]));
+        const float len = top - bot;
+        if (len > 0.f)
+            visit(((unsigned)ii[2] * ny + (unsigned)ii[1]) * nx + (unsigned)ii[0], len);
+        // axis with the nearest exit (first minimum on ties, as the oracle)
+        int m = -1;
+        float best = kInf;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (stp[c] != 0 && hi[c] < best) {
+                best = hi[c];
+                m = c;
+            }
+        if (m < 0 || best >= aout)
+            break;
+        // static indices only (no dynamically indexed register arrays)
+        bool leave = false;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == m) {
+                ii[c] += stp[c];
+                const int k = stp[c] > 0 ? ii[c] + 1 : ii[c];
+                lo[c] = hi[c];
+                hi[c] = plane_a(r, c, k);
+                leave = ii[c] < 0 || ii[c] >= r.n[c];
+            }
+        }
+        if (leave)
+            break;
+    }
+}
+
+template <int EP>
+__global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
+                                                   const float *__restrict__ data)
+{
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (u >= g.nu || v >= g.nv)
+        return;
+    RayA r;
+    ray_setup_a(g, p, u, v, r);
+    const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
+    if (EP == KV_EP_COUNT) {
+        int c = 0;
+        walk_a(r, [&](unsigned, float) { ++c; });
+        out[o] = (float)c;
+        return;
+    }
+    float acc = 0.f;
+    walk_a(r, [&](unsigned k, float len) { acc += len * vol[k]; });
+    const float ax = r.L * acc;
+    if (EP == KV_EP_PLAIN)
+        out[o] = ax;
+    else  // MLEM ratio on line-integral data
+        out[o] = ax > 0.f ? data[o] / ax : 0.f;
+}
+
+// Records the accepted pairs of sampled rays through the same walks.
+__global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__restrict__ rays, int cap,
+                             int64_t *__restrict__ idx, float *__restrict__ w, int32_t *__restrict__ counts)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrays)
+        return;
+    const int id = rays[t];
+    const int per = g.nu * g.nv;
+    const int p = id / per, rem = id - p * per, v = rem / g.nu, u = rem - v * g.nu;
+    int c = 0;
+    auto rec = [&](unsigned k, float wt) {
+        if (c < cap) {
+            idx[(size_t)t * cap + c] = (int64_t)k;
+            w[(size_t)t * cap + c] = wt;
+        }
+        ++c;
+    };
+    if (mode == KV_MODE_R) {
+        RayF r;
+        ray_setup(g, p, u, v, r);
+        walk_r(g, r, rec);
+    } else {
+        RayA r;
+        ray_setup_a(g, p, u, v, r);
+        walk_a(r, rec);
+    }
+    counts[t] = c;
+}
+
+}  // namespace
+
+hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data, float relax,
+                          hipStream_t s)
+{
+    const dim3 grid((g.nu + 63) / 64, (g.nv + 3) / 4, g.np), block(256);
+    switch (epilogue) {
+    case KV_EP_PLAIN: fp_r_kernel<KV_EP_PLAIN><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
+    case KV_EP_MLEM: fp_r_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
+    case KV_EP_COUNT: fp_r_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
+    default: fp_r_kernel<KV_EP_SART><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_fp_a(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data, hipStream_t s)
+{
+    const dim3 grid((g.nu + 63) / 64, (g.nv + 3) / 4, g.np), block(256);
+    if (epilogue == KV_EP_PLAIN)
+        fp_a_kernel<KV_EP_PLAIN><<<grid, block, 0, s>>>(g, vol, out, data);
+    else if (epilogue == KV_EP_COUNT)
+        fp_a_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, vol, out, data);
+    else
+        fp_a_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, vol, out, data);
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx, float *w,
+                           int32_t *counts, hipStream_t s)
+{
+    trace_kernel<<<(nrays + 63) / 64, 64, 0, s>>>(g, mode, nrays, rays, cap, idx, w, counts);
+    return hipGetLastError();
+}

@@ -1,0 +1,94 @@
+// Internal declarations shared by the HIP sources of libkevernals_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kevernals_amd.h"
+
+// Geometry as the kernels see it: passed by value as a kernel argument.
+// Per-view arrays live in device memory (det/src: np*3 floats each).
+struct KvGeom {
+    int nx, ny, nz;
+    int nu, nv, np;
+    float sx, sy, sz;
+    float pu, pv;
+    const float *det;
+    const float *src;
+};
+
+// Per-view plane-crossing tables.  For mode R an entry is the reference's
+// alpha1/alpha2 of Projector.cu:653-669 for plane k of a pixel's ray:
+//   x[p][u][k] = (|rx| > 1e-24) ? ((k - nx/2) - sx) / rx : 0
+// For mode A it is the mode-A plane crossing ((k - n/2) - s) * (1/d), with
+// +-INF encoding of rays parallel to the planes, plus the squared mm
+// components of each ray (lx2[p][u], ly2[p][v], lz2[p]).
+struct KvTables {
+    float *x = nullptr;   // [np][nu][nx+1]
+    float *y = nullptr;   // [np][nv][ny+1]
+    float *z = nullptr;   // [np][nz+1]
+    float *lx2 = nullptr; // mode A: [np][nu]
+    float *ly2 = nullptr; // mode A: [np][nv]
+    float *lz2 = nullptr; // mode A: [np]
+};
+
+// Epilogues of the projection kernels: plain result, fused MLEM ratio/update,
+// fused SART residual/update, or the count of accepted (ray, voxel) pairs.
+enum KvEpilogue { KV_EP_PLAIN = 0, KV_EP_MLEM = 1, KV_EP_SART = 2, KV_EP_COUNT = 3 };
+
+struct kv_context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool has_geom = false;
+    KvGeom g{};
+    float *d_det = nullptr;
+    float *d_src = nullptr;
+    KvTables tab_r{};
+    bool tab_r_ok = false;
+    KvTables tab_a{};
+    bool tab_a_ok = false;
+    float *d_ratio = nullptr;  // projection-sized work buffer of fused iterations
+    float *d_sens = nullptr;   // mode-A sensitivity A^T 1
+    bool sens_ok = false;
+    double *d_red = nullptr;   // partial sums of deterministic reductions
+    int red_cap = 0;
+};
+
+// Thread-local error message.
+void kv_set_error(const char *fmt, ...);
+
+#define KV_TRY_HIP(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t kv_e_ = (expr);                                                              \
+        if (kv_e_ != hipSuccess) {                                                              \
+            kv_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(kv_e_), __FILE__, \
+                         __LINE__);                                                             \
+            return KV_ERR_DEVICE;                                                               \
+        }                                                                                       \
+    } while (0)
+
+inline int64_t kv_nvox(const KvGeom &g) { return (int64_t)g.nx * g.ny * g.nz; }
+inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
+
+// ---- launchers (kv_fp.hip, kv_bp.hip, kv_misc.hip) ----------------------
+hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data,
+                          float relax, hipStream_t s);
+hipError_t kv_launch_fp_a(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data,
+                          hipStream_t s);
+hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx,
+                           float *w, int32_t *counts, hipStream_t s);
+hipError_t kv_launch_tables_r(const KvGeom &g, const KvTables &t, hipStream_t s);
+hipError_t kv_launch_tables_a(const KvGeom &g, const KvTables &t, hipStream_t s);
+hipError_t kv_launch_bp_r(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
+                          hipStream_t s);
+hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
+                          const float *sens, bool unit_input, hipStream_t s);
+hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes,
+                         float *out, hipStream_t s);
+hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s);
+hipError_t kv_launch_loglik(const KvGeom &g, const float *data, const float *ax, double *partials, int nblocks,
+                            hipStream_t s);

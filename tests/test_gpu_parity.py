@@ -1,0 +1,306 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on
+the same seeded inputs.
+
+Bars: mode-R voxel index sequences and weights bit-exact; mode-R ray sums and
+backprojections within 1e-5 relative (they are in fact bit-exact: same FP32
+operation sequence); fused MLEM/SART iterations bit-exact against the
+reference's composed host loops; mode A within FP32 summation-order
+tolerance (rtol 1e-5) and adjoint to 1e-6; shift-and-add bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import rng, tiny_geometry
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def K():
+    import kevernals_amd
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return kevernals_amd
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def small():
+    from kevernals_amd import geometry as G
+
+    return G.small_plain()
+
+
+def half_small():
+    from kevernals_amd import geometry as G
+
+    return G.scaled_down(G.small_plain(), 2)
+
+
+def phantom_of(g, seed):
+    from kevernals_amd.phantom import phantom
+
+    return phantom(g, seed)
+
+
+def report(name, a, b):
+    m = b != 0
+    rel = np.abs(a - b)[m] / np.abs(b[m]) if m.any() else np.zeros(1)
+    exact = float(np.mean(a == b))
+    print(f"[parity] {name}: exact {exact:.6f}  max rel {rel.max():.3e}", flush=True)
+    return exact, float(rel.max())
+
+
+# ------------------------------------------------------------ mode R -------
+def test_fp_r_small_config_parity(K, O):
+    g = small()
+    mu = phantom_of(g, 1)
+    ctx = K.Context(0, g)
+    out = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(dev(mu), out, K.MODE_R)
+    got = host(out)
+    ref = O.Oracle(g).fp_r(mu)
+    exact, rel = report("fp_r 128x128x32/15", got, ref)
+    assert rel <= 1e-5
+    assert exact == 1.0  # same FP32 operation sequence
+
+
+def test_fp_r_voxel_sequences_bit_exact(K, O):
+    g = small()
+    ctx = K.Context(0, g)
+    o = O.Oracle(g)
+    r = rng(21)
+    rays = r.integers(0, g.nrays, size=3000).astype(np.int32)
+    traces = ctx.trace_rays(rays, cap=g.nx + g.ny + g.nz + 3, mode=K.MODE_R)
+    nonempty = 0
+    for rid, (idx, w, cnt) in zip(rays.tolist(), traces):
+        p, rem = divmod(rid, g.nu * g.nv)
+        v, u = divmod(rem, g.nu)
+        ri, rw = o.ray_trace(p, u, v)
+        assert cnt == len(ri), (p, u, v)
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_array_equal(w, rw)
+        nonempty += cnt > 0
+    assert nonempty > 500
+
+
+def test_bp_r_small_config_parity(K, O):
+    g = small()
+    proj = (0.5 + rng(22).random(g.proj_shape)).astype(np.float32)
+    ctx = K.Context(0, g)
+    out = torch.empty(g.vol_shape, device="cuda")
+    ctx.back_project(dev(proj), out, K.MODE_R)
+    got = host(out)
+    z0, z1 = 12, 16  # bounded oracle work: four full slices at the real size
+    ref = O.Oracle(g).bp_r_slices(proj, z0, z1)
+    exact, rel = report("bp_r 128x128x32/15 slices 12-15", got[z0:z1], ref[z0:z1])
+    assert rel <= 1e-5
+    assert exact == 1.0
+
+
+def test_bp_r_half_config_full_parity(K, O):
+    g = half_small()
+    proj = (0.5 + rng(23).random(g.proj_shape)).astype(np.float32)
+    ctx = K.Context(0, g)
+    out = torch.empty(g.vol_shape, device="cuda")
+    ctx.back_project(dev(proj), out, K.MODE_R)
+    exact, rel = report("bp_r 64x64x16/15", host(out), O.Oracle(g).bp_r(proj))
+    assert rel <= 1e-5 and exact == 1.0
+
+
+def test_fused_mlem_matches_reference_loop_bit_exact(K, O):
+    g = half_small()
+    o = O.Oracle(g)
+    mu = phantom_of(g, 2)
+    data = o.fp_r(mu)
+    x0 = np.ones(g.vol_shape, np.float32)
+    ctx = K.Context(0, g)
+    x = dev(x0)
+    d = dev(data)
+    for _ in range(3):
+        ctx.mlem_iteration(x, d, 1.0, K.MODE_R)
+    ref = o.mlem(data, x0, 3, 1.0)
+    exact, rel = report("mlem_r x3 64x64x16", host(x), ref)
+    assert exact == 1.0
+
+
+def test_fused_art_matches_reference_loop_bit_exact(K, O):
+    g = half_small()
+    o = O.Oracle(g)
+    mu = phantom_of(g, 3)
+    data = o.fp_r(mu)
+    x0 = np.full(g.vol_shape, 0.01, np.float32)
+    ctx = K.Context(0, g)
+    x = dev(x0)
+    for _ in range(3):
+        ctx.sart_iteration(x, dev(data), 0.5)
+    ref = o.art(data, x0, 3, 0.5)
+    exact, rel = report("art x3 64x64x16", host(x), ref)
+    assert exact == 1.0
+
+
+def test_fp_r_early_exit_quirk(K, O):
+    """An axis-parallel ray returns 0 (Projector.cu:79-85)."""
+    from kevernals_amd.geometry import Geometry
+
+    det = np.array([[-4.0, -4.0, -30.0]], np.float32)
+    src = np.array([[-3.5, 0.0, 50.0]], np.float32)
+    g = Geometry(4, 4, 4, 1.0, 1.0, 1.0, 8, 8, 1, 1.0, 1.0, det, src)
+    ctx = K.Context(0, g)
+    out = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(dev(np.ones(g.vol_shape, np.float32)), out, K.MODE_R)
+    got = host(out)
+    assert got[0, 3, 0] == 0.0
+    np.testing.assert_array_equal(got, O.Oracle(g).fp_r(np.ones(g.vol_shape, np.float32)))
+
+
+def test_tiny_geometries_edge_cases(K, O):
+    """Odd sizes (not multiples of the 64x4 tiles), off-centre detector."""
+    for vol, det in (((6, 5, 4), (12, 10)), ((7, 9, 3), (13, 5)), ((1, 1, 1), (3, 3))):
+        g = tiny_geometry(nviews=3, vol=vol, det=det)
+        o = O.Oracle(g)
+        x = (0.1 + rng(24).random(g.vol_shape)).astype(np.float32)
+        y = (0.1 + rng(25).random(g.proj_shape)).astype(np.float32)
+        ctx = K.Context(0, g)
+        fp = torch.empty(g.proj_shape, device="cuda")
+        bp = torch.empty(g.vol_shape, device="cuda")
+        ctx.forward_project(dev(x), fp, K.MODE_R)
+        ctx.back_project(dev(y), bp, K.MODE_R)
+        np.testing.assert_array_equal(host(fp), o.fp_r(x))
+        np.testing.assert_array_equal(host(bp), o.bp_r(y))
+        ctx.forward_project(dev(x), fp, K.MODE_A)
+        ctx.back_project(dev(y), bp, K.MODE_A)
+        np.testing.assert_allclose(host(fp), o.fp_a(x), rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(host(bp), o.bp_a(y), rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------ mode A -------
+def test_mode_a_parity_and_adjoint(K, O):
+    g = half_small()
+    o = O.Oracle(g)
+    r = rng(26)
+    x = r.random(g.vol_shape).astype(np.float32)
+    y = r.random(g.proj_shape).astype(np.float32)
+    ctx = K.Context(0, g)
+    fp = torch.empty(g.proj_shape, device="cuda")
+    bp = torch.empty(g.vol_shape, device="cuda")
+    ctx.forward_project(dev(x), fp, K.MODE_A)
+    ctx.back_project(dev(y), bp, K.MODE_A)
+    Ax, Aty = host(fp), host(bp)
+    report("fp_a 64x64x16", Ax, o.fp_a(x))
+    report("bp_a 64x64x16", Aty, o.bp_a(y))
+    np.testing.assert_allclose(Ax, o.fp_a(x), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(Aty, o.bp_a(y), rtol=1e-5, atol=1e-5)
+    lhs = float(np.dot(Ax.ravel().astype(np.float64), y.ravel()))
+    rhs = float(np.dot(x.ravel().astype(np.float64), Aty.ravel()))
+    print(f"[adjoint] <Ax,y>={lhs:.9e} <x,A^T y>={rhs:.9e} rel={abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+def test_mode_a_sequences_match_oracle(K, O):
+    g = half_small()
+    ctx = K.Context(0, g)
+    o = O.Oracle(g)
+    rays = rng(27).integers(0, g.nrays, size=1000).astype(np.int32)
+    for rid, (idx, w, cnt) in zip(rays.tolist(), ctx.trace_rays(rays, cap=512, mode=K.MODE_A)):
+        p, rem = divmod(rid, g.nu * g.nv)
+        v, u = divmod(rem, g.nu)
+        ri, rw = o.ray_trace(p, u, v, mode="A")
+        np.testing.assert_array_equal(idx, ri)
+        np.testing.assert_allclose(w, rw, rtol=1e-6, atol=1e-7)
+
+
+def test_mlem_a_against_fp64_oracle_and_loglik(K, O):
+    g = half_small()
+    o = O.Oracle(g)
+    from kevernals_amd.phantom import poisson_line_integrals
+
+    mu = phantom_of(g, 4)
+    data = poisson_line_integrals(o.fp_a(mu), 1e4, 5)
+    ctx = K.Context(0, g)
+    x = dev(np.ones(g.vol_shape, np.float32))
+    d = dev(data)
+    lls = [ctx.log_likelihood(d, x)]
+    for _ in range(5):
+        ctx.mlem_iteration(x, d, 1.0, K.MODE_A)
+        lls.append(ctx.log_likelihood(d, x))
+    ref = o.mlem_a64(data, np.ones(g.vol_shape), 5)
+    got = host(x)
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"[parity] mlem_a x5 vs fp64 oracle: max rel {rel:.2e}")
+    assert rel < 1e-4
+    ll_ref = o.loglik_a64(data.astype(np.float64), got.astype(np.float64))
+    assert abs(lls[-1] - ll_ref) <= 1e-5 * abs(ll_ref)
+    assert all(b >= a - 1e-7 * abs(a) for a, b in zip(lls, lls[1:]))
+
+
+# ---------------------------------------------------------- shift-and-add --
+def test_shift_and_add_bit_exact(K, O):
+    g = small()
+    proj = rng(28).random(g.proj_shape).astype(np.float32)
+    ctx = K.Context(0, g)
+    planes = g.plane_heights()
+    sid = float(abs(g.det[0, 2] - np.float32(g.src[:, 2].mean(dtype=np.float32))))
+    out = torch.empty((len(planes), g.nv, g.nu), device="cuda")
+    ctx.shift_and_add(dev(proj), out, g.src[:, 1], planes, sid, 0.0, g.sy)
+    rc, ref = O.saa(proj, g.src[:, 1], planes, sid, 0.0, g.sy)
+    assert rc == 0
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_shift_and_add_preconditions(K):
+    g = small()
+    ctx = K.Context(0, g)
+    proj = torch.zeros(g.proj_shape, device="cuda")
+    out = torch.zeros((2, g.nv, g.nu), device="cuda")
+    with pytest.raises(K.KvError) as e:
+        ctx.shift_and_add(proj, out, g.src[:, 1], [1.0], 1000.0, 0.0, g.sy)
+    assert e.value.status == 5
+    with pytest.raises(K.KvError) as e:
+        ctx.shift_and_add(proj, out, g.src[:, 1], [1.0, 2.0], 1000.0, 1000.0, g.sy)
+    assert e.value.status == 5
+
+
+# ----------------------------------------------------------- error paths --
+def test_error_statuses(K):
+    ctx = K.Context(0)
+    with pytest.raises(K.KvError) as e:
+        ctx.forward_project(1, 1, K.MODE_R)
+    assert e.value.status == 4  # no geometry
+    from kevernals_amd.geometry import Geometry
+
+    bad = Geometry(0, 4, 4, 1.0, 1.0, 1.0, 8, 8, 1, 1.0, 1.0, np.zeros((1, 3)), np.ones((1, 3)))
+    with pytest.raises(K.KvError) as e:
+        ctx.set_geometry(bad)
+    assert e.value.status == 1
+    ctx.set_geometry(tiny_geometry())
+    with pytest.raises(K.KvError) as e:
+        ctx.forward_project(1, 1, 7)
+    assert e.value.status == 1
+
+
+def test_host_variants_match_device(K):
+    g = half_small()
+    ctx = K.Context(0, g)
+    x = rng(29).random(g.vol_shape).astype(np.float32)
+    fp = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(dev(x), fp, K.MODE_R)
+    np.testing.assert_array_equal(ctx.forward_project_host(x), host(fp))
+    y = host(fp)
+    bp = torch.empty(g.vol_shape, device="cuda")
+    ctx.back_project(dev(y), bp, K.MODE_R)
+    np.testing.assert_array_equal(ctx.back_project_host(y), host(bp))

@@ -1,0 +1,386 @@
+"""Pins of the CPU oracle (oracle/) against what the mathematics fixes.
+
+The reference ships no golden vectors, fixtures or known-answer tests for the
+projection path (its only tests cover the Dim2/Dim3 value types), and its
+CUDA code cannot be built here.  The oracle is therefore pinned by checks that
+a plausible slip (dropped term, wrong sign or index, transposed operand)
+would fail:
+
+* exact rational ray/voxel-box intersections (slab clipping with Fractions,
+  an algorithm different from the crossing-list merge being tested);
+* closed forms (uniform volumes, chord lengths, mean images);
+* invariants (weighted means, linearity, mirror symmetry, adjointness,
+  fixed points, monotone likelihood);
+* the reference's documented quirks reproduced on purpose.
+"""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+from conftest import rng, tiny_geometry
+from kevernals_amd import geometry as G
+
+
+# ---------------------------------------------------------------- helpers --
+def ray_f(g, p, u, v):
+    """Source and direction of ray (p,u,v) in floating-voxel units, evaluated
+    in FP32 as the reference does (Projector.cu:303-317); returned exactly."""
+    f = np.float32
+    S, D = g.src[p], g.det[p]
+    s = [f(S[0]) / f(g.sx), f(S[1]) / f(g.sy), f(S[2]) / f(g.sz)]
+    q = [(f(D[0]) + (f(u) + f(0.5)) * f(g.pu)) / f(g.sx),
+         (f(D[1]) + (f(v) + f(0.5)) * f(g.pv)) / f(g.sy),
+         f(D[2]) / f(g.sz)]
+    d = [f(q[i] - s[i]) for i in range(3)]
+    return [Fr(float(x)) for x in s], [Fr(float(x)) for x in d]
+
+
+def box_interval(s, d, lo, hi):
+    """Exact alpha interval of s + a*d inside box [lo, hi], clipped to [0,1]."""
+    t0, t1 = Fr(0), Fr(1)
+    for c in range(3):
+        if d[c] == 0:
+            if not (lo[c] < s[c] < hi[c]):
+                return None
+            continue
+        a, b = (lo[c] - s[c]) / d[c], (hi[c] - s[c]) / d[c]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    return (t0, t1) if t1 > t0 else None
+
+
+def voxel_box(g, k):
+    x = k % g.nx
+    y = (k // g.nx) % g.ny
+    z = k // (g.nx * g.ny)
+    lo = [Fr(x) - Fr(g.nx, 2), Fr(y) - Fr(g.ny, 2), Fr(z) - Fr(g.nz, 2)]
+    return lo, [c + 1 for c in lo], (x, y, z)
+
+
+def voxel_len(g, s, d, k):
+    lo, hi, _ = voxel_box(g, k)
+    iv = box_interval(s, d, lo, hi)
+    return 0 if iv is None else iv[1] - iv[0]
+
+
+def hitting_rays(o, g, count, seed=0):
+    r = rng(seed)
+    out = []
+    tries = 0
+    while len(out) < count and tries < 20000:
+        tries += 1
+        p, u, v = int(r.integers(g.nviews)), int(r.integers(g.nu)), int(r.integers(g.nv))
+        idx, _ = o.ray_trace(p, u, v)
+        if len(idx) >= 6:
+            out.append((p, u, v))
+    return out
+
+
+# ------------------------------------------------------ AlmostEqualRelative --
+def test_almost_equal_relative_closed_forms(oracle_mod):
+    aer = oracle_mod.almost_equal_relative
+    eps = float(np.finfo(np.float32).eps)
+    assert aer(0.0, 0.0)
+    assert not aer(1e-30, 0.0)  # only exact zero is "almost zero"
+    assert aer(1.0, 1.0 + eps)
+    assert not aer(1.0, 1.0 + 4 * eps)
+    assert not aer(float("nan"), 0.0)
+    assert aer(650.0, 650.0)
+
+
+# ------------------------------------------------ A1: the crossing list -----
+def test_crossing_list_structure(oracle_mod):
+    """Sorted; length (hi-lo)+3 summed over axes; x and y runs hold every
+    crossed plane strictly inside the volume; the z run lacks the last two
+    planes in traversal order (Projector.cu:182); 1.0 fillers remain."""
+    g = tiny_geometry(nviews=3, vol=(8, 7, 6))
+    o = oracle_mod.Oracle(g, 1)
+    for p, u, v in hitting_rays(o, g, 25):
+        a = o.ray_alphas(p, u, v).astype(np.float64)
+        assert np.all(np.diff(a) >= 0), "list not ascending"
+        s, d = ray_f(g, p, u, v)
+        full = box_interval(s, d, [-Fr(g.nx, 2), -Fr(g.ny, 2), -Fr(g.nz, 2)],
+                           [Fr(g.nx, 2), Fr(g.ny, 2), Fr(g.nz, 2)])
+        assert full is not None
+        a0, a1 = full
+        # every interior x / y plane crossed inside the volume is in the list
+        for c, n in ((0, g.nx), (1, g.ny)):
+            for k in range(1, n):
+                t = (Fr(k) - Fr(n, 2) - s[c]) / d[c]
+                if a0 + Fr(1, 10**4) < t < a1 - Fr(1, 10**4):
+                    assert np.min(np.abs(a - float(t))) < 2e-5, (c, k)
+        # z planes 0..nz-1 (plane nz is clamped away, Projector.cu:113-119)
+        # crossed inside the volume, except the last two in traversal order
+        zc = sorted((Fr(k) - Fr(g.nz, 2) - s[2]) / d[2] for k in range(0, g.nz))
+        zc = [t for t in zc if a0 - Fr(1, 10**4) <= t <= a1 + Fr(1, 10**4)]
+        for t in zc[:-2]:
+            assert np.min(np.abs(a - float(t))) < 2e-5
+        # the list has exactly two more 1.0 entries than planes at alpha 1.0
+        # when the ray spans >= 2 z intervals (the quirk's fillers)
+        if len(zc) >= 3:
+            assert np.count_nonzero(a == 1.0) >= 2
+
+
+def test_segment_weights_equal_exact_box_lengths(oracle_mod):
+    """Per-voxel summed alpha-weights of the forward walk equal the exact
+    rational ray/box intersection lengths away from the entry/exit voxels and
+    the two bottom z layers that the truncated z run merges."""
+    g = tiny_geometry(nviews=3, vol=(8, 7, 6))
+    o = oracle_mod.Oracle(g, 1)
+    checked = 0
+    for p, u, v in hitting_rays(o, g, 30, seed=1):
+        idx, w = o.ray_trace(p, u, v)
+        s, d = ray_f(g, p, u, v)
+        order = list(dict.fromkeys(idx.tolist()))
+        zs = [k // (g.nx * g.ny) for k in order]
+        zlast = list(dict.fromkeys(zs))[-2:]
+        W = {}
+        for k, wt in zip(idx.tolist(), w.tolist()):
+            W[k] = W.get(k, 0.0) + wt
+        for k in order[1:-1]:
+            if k // (g.nx * g.ny) in zlast:
+                continue
+            assert abs(W[k] - float(voxel_len(g, s, d, k))) < 2e-6, (p, u, v, k)
+            checked += 1
+    assert checked > 50
+
+
+def test_quirk_axis_parallel_ray_is_empty(oracle_mod):
+    """|d_x| <= 1e-6 makes alpha1 = alphaN = 0, so alphaMax = 0 and the
+    reference exits early (Projector.cu:55-61,79-85): value 0."""
+    from kevernals_amd.geometry import Geometry
+
+    det = np.array([[-4.0, -4.0, -30.0]], np.float32)
+    src = np.array([[-3.5, 0.0, 50.0]], np.float32)  # above pixel u=0 centre
+    g = Geometry(4, 4, 4, 1.0, 1.0, 1.0, 8, 8, 1, 1.0, 1.0, det, src)
+    o = oracle_mod.Oracle(g, 1)
+    assert len(o.ray_alphas(0, 0, 3)) == 0
+    assert o.fp_r(np.ones(g.vol_shape, np.float32))[0, 3, 0] == 0.0
+
+
+# --------------------------------------------------- A2: forward operator ---
+def test_fp_r_uniform_volume_is_constant(oracle_mod):
+    """Weighted mean of a constant is the constant (Projector.cu:368-371)."""
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    out = o.fp_r(np.full(g.vol_shape, 0.37, np.float32))
+    hit = out != 0
+    assert hit.mean() > 0.2
+    np.testing.assert_allclose(out[hit], 0.37, rtol=3e-7)
+
+
+def test_fp_r_linear_and_fp64_twin(oracle_mod):
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    r = rng(3)
+    x = r.random(g.vol_shape).astype(np.float32)
+    y = r.random(g.vol_shape).astype(np.float32)
+    lhs = o.fp_r(2.0 * x + 3.0 * y)
+    rhs = 2.0 * o.fp_r(x) + 3.0 * o.fp_r(y)
+    np.testing.assert_allclose(lhs, rhs, rtol=2e-5, atol=1e-6)
+    # FP32 vs the FP64 twin: the crossing list is built with truncations
+    # (Projector.cu:89-120), so short corner-clipping rays can change their
+    # plane ranges between precisions.  Stated error: median ~1e-6, 99% of
+    # rays within 2e-5.
+    a, b = o.fp_r(x), o.fp_r64(x)
+    m = b != 0
+    rel = np.abs(a[m] - b[m]) / np.abs(b[m])
+    assert np.median(rel) < 3e-6 and np.quantile(rel, 0.99) < 2e-5
+
+
+def test_mirror_symmetry(oracle_mod):
+    """Mirroring the volume in y and the scan (view order, detector rows)
+    mirrors mode-A projections.  Mode R is not mirror-symmetric by
+    construction: the far face plane is clamped away (Projector.cu:91-119)
+    while plane 0 is kept, and indices truncate toward zero
+    (Projector.cu:352-354); the test records that asymmetry."""
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    x = rng(4).random(g.vol_shape).astype(np.float32)
+    xm = np.ascontiguousarray(x[:, ::-1, :])
+    a, b = o.fp_a(x), o.fp_a(xm)[::-1, ::-1, :]
+    assert np.isclose(a, b, rtol=1e-4, atol=1e-6).mean() > 0.9999
+    ar, br = o.fp_r(x), o.fp_r(xm)[::-1, ::-1, :]
+    frac = np.isclose(ar, br, rtol=1e-4, atol=1e-6).mean()
+    assert 0.5 < frac < 1.0
+
+
+# ------------------------------------------------------ A3: backprojector ---
+def test_bp_r_uniform_projections_is_constant(oracle_mod):
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    out = o.bp_r(np.full(g.proj_shape, 1.25, np.float32))
+    hit = out != 0
+    assert hit.mean() > 0.9
+    np.testing.assert_allclose(out[hit], 1.25, rtol=1e-6)  # a few ulp of FP32 summation
+
+
+def test_bp_r_weights_are_exact_box_lengths(oracle_mod):
+    """Each accepted (pixel, voxel) weight is the exact alpha-length of the
+    pixel ray inside the voxel box (Projector.cu:653-712); every ray that
+    crosses the box by more than 1e-4 within 4 pixels of the voxel centre's
+    projection is accepted (the +-5 window holds it)."""
+    g = tiny_geometry(nviews=3, vol=(6, 5, 4), det=(24, 20), pitch=(0.8, 0.9))
+    o = oracle_mod.Oracle(g, 1)
+    for (x, y, z) in [(2, 2, 1), (0, 4, 3), (5, 0, 0), (3, 1, 2)]:
+        pix, w = o.bp_voxel_trace(x, y, z)
+        assert len(pix) > 0
+        k = (z * g.ny + y) * g.nx + x
+        for pi, wt in zip(pix.tolist(), w.tolist()):
+            p, rem = divmod(pi, g.nu * g.nv)
+            v, u = divmod(rem, g.nu)
+            s, d = ray_f(g, p, u, v)
+            assert abs(float(voxel_len(g, s, d, k)) - wt) < 1e-6
+        acc = set(pix.tolist())
+        for p in range(g.nviews):
+            S = g.src[p].astype(np.float64)
+            C = np.array([(x + 0.5 - g.nx / 2) * g.sx, (y + 0.5 - g.ny / 2) * g.sy, (z + 0.5 - g.nz / 2) * g.sz])
+            t = (g.det[p, 2] - S[2]) / (C[2] - S[2])
+            uc = (S[0] + (C[0] - S[0]) * t - g.det[p, 0]) / g.pu
+            vc = (S[1] + (C[1] - S[1]) * t - g.det[p, 1]) / g.pv
+            for v in range(max(0, int(vc) - 4), min(g.nv, int(vc) + 5)):
+                for u in range(max(0, int(uc) - 4), min(g.nu, int(uc) + 5)):
+                    s, d = ray_f(g, p, u, v)
+                    if voxel_len(g, s, d, k) > Fr(1, 10**4):
+                        assert (p * g.nv + v) * g.nu + u in acc
+
+
+def test_bp_r_window_is_clamped_at_the_detector_edge(oracle_mod):
+    """A voxel whose centre projects next to the detector's first column sees
+    columns 0..5 at most (window clamped per end, Projector.cu:609-646)."""
+    from kevernals_amd.geometry import Geometry
+
+    det = np.array([[0.0, -8.0, -30.0]], np.float32)   # detector starts at x = 0
+    src = np.array([[0.0, 0.0, 60.0]], np.float32)
+    g = Geometry(8, 8, 2, 1.0, 1.0, 1.0, 32, 16, 1, 0.5, 1.0, det, src)
+    o = oracle_mod.Oracle(g, 1)
+    pix, w = o.bp_voxel_trace(4, 4, 0)  # centre x = 0.5 -> just right of the edge
+    us = [pi % g.nu for pi in pix.tolist()]
+    assert len(us) > 0 and max(us) <= 5 + 2 and min(us) >= 0
+
+
+# ------------------------------------------------------------ mode A ------
+def test_mode_a_walk_equals_brute_force(oracle_mod):
+    g = tiny_geometry(nviews=4, vol=(7, 6, 5), det=(14, 12))
+    o = oracle_mod.Oracle(g, 4)
+    x = rng(5).random(g.vol_shape).astype(np.float32)
+    np.testing.assert_allclose(o.fp_a(x), o.fp_a(x, brute=True), rtol=1e-6, atol=1e-7)
+
+
+def test_mode_a_lengths_exact(oracle_mod):
+    g = tiny_geometry(nviews=3, vol=(7, 6, 5), det=(14, 12))
+    o = oracle_mod.Oracle(g, 1)
+    r = rng(6)
+    n = 0
+    for _ in range(200):
+        p, u, v = int(r.integers(g.nviews)), int(r.integers(g.nu)), int(r.integers(g.nv))
+        idx, ln = o.ray_trace(p, u, v, mode="A")
+        s, d = ray_f(g, p, u, v)
+        got = dict(zip(idx.tolist(), ln.tolist()))
+        for k in range(g.nvox):
+            e = float(voxel_len(g, s, d, k))
+            if e > 1e-6:
+                assert k in got
+                n += 1
+            assert abs(got.get(k, 0.0) - e) < 1e-6
+    assert n > 100
+
+
+def test_mode_a_uniform_volume_gives_chord_lengths(oracle_mod):
+    g = tiny_geometry(nviews=3, vol=(7, 6, 5), det=(14, 12))
+    o = oracle_mod.Oracle(g, 4)
+    out = o.fp_a(np.ones(g.vol_shape, np.float32))
+    lo = [-Fr(g.nx, 2), -Fr(g.ny, 2), -Fr(g.nz, 2)]
+    hi = [-c for c in lo]
+    for p in range(g.nviews):
+        for v in range(g.nv):
+            for u in range(g.nu):
+                s, d = ray_f(g, p, u, v)
+                iv = box_interval(s, d, lo, hi)
+                L = np.sqrt((float(d[0]) * g.sx) ** 2 + (float(d[1]) * g.sy) ** 2 + (float(d[2]) * g.sz) ** 2)
+                e = 0.0 if iv is None else float(iv[1] - iv[0]) * L
+                assert abs(out[p, v, u] - e) <= 2e-5 * max(1.0, e)
+
+
+def test_mode_a_adjoint(oracle_mod):
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    r = rng(7)
+    x = r.random(g.vol_shape)
+    y = r.random(g.proj_shape)
+    lhs = float(np.dot(o.fp_a64(x).ravel(), y.ravel()))
+    rhs = float(np.dot(x.ravel(), o.bp_a64(y).ravel()))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    lhs32 = float(np.dot(o.fp_a(x).ravel().astype(np.float64), y.astype(np.float32).ravel()))
+    rhs32 = float(np.dot(x.astype(np.float32).ravel().astype(np.float64), o.bp_a(y).ravel()))
+    assert abs(lhs32 - rhs32) <= 1e-6 * abs(lhs32)
+
+
+# ------------------------------------------------- A7 / A8 / A10 loops -----
+def test_art_fixed_point_is_exact(oracle_mod):
+    """Consistent data p = FP(x*): zero residual, x* unchanged bit for bit."""
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    x = (0.02 + 0.03 * rng(8).random(g.vol_shape)).astype(np.float32)
+    data = o.fp_r(x)
+    np.testing.assert_array_equal(o.art(data, x, 2, 0.5), x)
+
+
+def test_mlem_fixed_point_where_all_rays_are_consistent(oracle_mod):
+    """With lambda = 1 and p = FP(x*), the ratio is exactly 1 where FP != 0;
+    voxels fed only by such rays keep x* bit for bit, the others (fed by some
+    zero ratios of rays the forward walk drops) can only shrink."""
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    x = (0.02 + 0.03 * rng(9).random(g.vol_shape)).astype(np.float32)
+    data = o.fp_r(x)
+    x1 = o.mlem(data, x, 1, 1.0)
+    zero_rays = (data == 0).astype(np.float32)
+    touched = o.bp_r(zero_rays) != 0
+    np.testing.assert_array_equal(x1[~touched], x[~touched])
+    assert np.all(x1[touched] <= x[touched])
+    assert (~touched).mean() > 0.3
+
+
+def test_mlem_a_increases_loglik_and_keeps_positivity(oracle_mod):
+    g = G.scaled_down(G.small_plain(), 4)
+    o = oracle_mod.Oracle(g, 8)
+    from kevernals_amd.phantom import phantom, poisson_line_integrals
+
+    mu = phantom(g, 11)
+    data = poisson_line_integrals(o.fp_a(mu), 1e4, 12).astype(np.float64)
+    x = np.ones(g.vol_shape)
+    ll = [o.loglik_a64(data, x)]
+    for _ in range(4):
+        x = o.mlem_a64(data, x, 1)
+        ll.append(o.loglik_a64(data, x))
+        assert np.all(x >= 0)
+    assert all(b >= a - 1e-9 * abs(a) for a, b in zip(ll, ll[1:]))
+    assert ll[-1] > ll[0]
+
+
+def test_saa_closed_forms(oracle_mod):
+    r = rng(10)
+    P, nv, nu = 5, 12, 9
+    proj = r.random((P, nv, nu)).astype(np.float32)
+    sid, vdy = 100.0, 0.5
+    # plane at the fulcrum: every shift is 0 -> the mean image
+    zf = 10.0
+    rc, out = oracle_mod.saa(proj, np.linspace(-20, 20, P), [zf, zf], sid, zf, vdy)
+    assert rc == 0
+    np.testing.assert_allclose(out[0], proj.mean(axis=0), rtol=1e-6)
+    # impulse: one pixel per view lands at v - t_p with weight 1/P
+    imp = np.zeros((P, nv, nu), np.float32)
+    imp[2, 6, 4] = 1.0
+    ys = np.array([-8.0, -4.0, 0.0, 4.0, 8.0], np.float32)
+    z = 30.0
+    rc, out = oracle_mod.saa(imp, ys, [z, 0.0], sid, 0.0, vdy)
+    mz = np.float32(sid) / (np.float32(sid) - np.float32(z))
+    mf = np.float32(sid) / (np.float32(sid) - np.float32(0.0))
+    t2 = int(np.float32(ys[2] * np.float32(mz - mf)) / np.float32(vdy))
+    assert rc == 0 and out[0, 6 - t2, 4] == np.float32(0.2) and np.count_nonzero(out[0]) == 1
+    # preconditions (Reconstructors.h:275-305,319-323)
+    assert oracle_mod.saa(proj, ys, [1.0], sid, 0.0, vdy)[0] == -1
+    assert oracle_mod.saa(proj[:1], ys[:1], [1.0, 2.0], sid, 0.0, vdy)[0] == -2
+    assert oracle_mod.saa(proj, ys, [1.0, 2.0], sid, sid, vdy)[0] == -3
+    assert oracle_mod.saa(proj, ys, [1.0, sid], sid, 0.0, vdy)[0] == -4
