@@ -65,8 +65,10 @@ typedef struct kv_context kv_context;
  * stream, the device copy of the geometry and its precomputed tables. */
 kv_status kv_create(int device, kv_context **out);
 kv_status kv_destroy(kv_context *ctx);
-/* Use an external HIP stream (hipStream_t) for all later work; NULL restores
- * the context's own stream. */
+/* Use an external HIP stream (hipStream_t) for all later work; NULL selects
+ * the null (default) stream.  A new context uses a stream of its own; a
+ * caller that allocates with a stream-ordered allocator (torch's caching
+ * allocator) must enqueue on the allocator's stream. */
 kv_status kv_set_stream(kv_context *ctx, void *hip_stream);
 void *kv_get_stream(kv_context *ctx);
 kv_status kv_synchronize(kv_context *ctx);

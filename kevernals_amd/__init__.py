@@ -131,16 +131,33 @@ def _host_ptr(a: np.ndarray) -> int:
 
 
 class Context:
-    """One kv_context: a device, a stream, a geometry and its tables."""
+    """One kv_context: a device, a stream, a geometry and its tables.
 
-    def __init__(self, device: int = 0, geometry: Optional[Geometry] = None):
+    With follow_torch_stream (the default) every device operation is
+    enqueued on torch's current stream, so buffers from torch's caching
+    allocator (including temporaries released right after the call) are
+    reused only after the kernels that read them."""
+
+    def __init__(self, device: int = 0, geometry: Optional[Geometry] = None, follow_torch_stream: bool = True):
         self.lib = load_library()
         h = _P()
         self._check(self.lib.kv_create(device, ctypes.byref(h)))
         self.handle = h
         self.geometry = None
+        self.follow_torch_stream = follow_torch_stream
+        self._stream = None
         if geometry is not None:
             self.set_geometry(geometry)
+
+    def _on_torch_stream(self):
+        if not self.follow_torch_stream:
+            return
+        import torch
+
+        s = torch.cuda.current_stream().cuda_stream
+        if s != self._stream:
+            self._check(self.lib.kv_set_stream(self.handle, s))
+            self._stream = s
 
     def _check(self, status: int):
         if status != KV_OK:
@@ -159,13 +176,16 @@ class Context:
 
     # -- plumbing ---------------------------------------------------------
     def set_stream(self, stream_handle: Optional[int]):
+        """Enqueue on an explicit hipStream_t (0 = null stream) from now on;
+        stops following torch's current stream."""
+        self.follow_torch_stream = False
         self._check(self.lib.kv_set_stream(self.handle, stream_handle or 0))
+        self._stream = stream_handle or 0
 
     def use_torch_stream(self):
-        """Enqueue on torch's current stream (so torch events time our work)."""
-        import torch
-
-        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        """Follow torch's current stream (the default)."""
+        self.follow_torch_stream = True
+        self._on_torch_stream()
 
     def synchronize(self):
         self._check(self.lib.kv_synchronize(self.handle))
@@ -179,43 +199,47 @@ class Context:
         self._check(self.lib.kv_set_geometry(self.handle, ctypes.byref(kg)))
         self.geometry = g
 
+    def _dev(self, fn, *args):
+        self._on_torch_stream()
+        self._check(fn(self.handle, *args))
+
     # -- operators (device buffers) ---------------------------------------
     def forward_project(self, vol, proj, mode: int = MODE_R):
-        self._check(self.lib.kv_forward_project(self.handle, _ptr(vol), _ptr(proj), mode))
+        self._dev(self.lib.kv_forward_project, _ptr(vol), _ptr(proj), mode)
 
     def back_project(self, proj, vol, mode: int = MODE_R):
-        self._check(self.lib.kv_back_project(self.handle, _ptr(proj), _ptr(vol), mode))
+        self._dev(self.lib.kv_back_project, _ptr(proj), _ptr(vol), mode)
 
     def mlem_iteration(self, vol, data, relax: float = 1.0, mode: int = MODE_R):
-        self._check(self.lib.kv_mlem_iteration(self.handle, _ptr(vol), _ptr(data), relax, mode))
+        self._dev(self.lib.kv_mlem_iteration, _ptr(vol), _ptr(data), relax, mode)
 
     def mlem_forward(self, vol, data, relax: float = 1.0, mode: int = MODE_R):
-        self._check(self.lib.kv_mlem_forward(self.handle, _ptr(vol), _ptr(data), relax, mode))
+        self._dev(self.lib.kv_mlem_forward, _ptr(vol), _ptr(data), relax, mode)
 
     def mlem_backward(self, vol, mode: int = MODE_R):
-        self._check(self.lib.kv_mlem_backward(self.handle, _ptr(vol), mode))
+        self._dev(self.lib.kv_mlem_backward, _ptr(vol), mode)
 
     def count_pairs(self, mode: int = MODE_R):
         """(forward pairs, backprojector pairs) accepted by the two passes."""
         a, b = ctypes.c_int64(), ctypes.c_int64()
-        self._check(self.lib.kv_count_pairs(self.handle, mode, ctypes.byref(a), ctypes.byref(b)))
+        self._dev(self.lib.kv_count_pairs, mode, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
     def sart_iteration(self, vol, data, relax: float = 0.5):
-        self._check(self.lib.kv_sart_iteration(self.handle, _ptr(vol), _ptr(data), relax))
+        self._dev(self.lib.kv_sart_iteration, _ptr(vol), _ptr(data), relax)
 
     def sensitivity(self, vol):
-        self._check(self.lib.kv_sensitivity(self.handle, _ptr(vol)))
+        self._dev(self.lib.kv_sensitivity, _ptr(vol))
 
     def shift_and_add(self, proj, out, src_y, plane_z, sid: float, fulcrum_z: float, voxel_dy: float):
         sy = np.ascontiguousarray(src_y, dtype=np.float32)
         pz = np.ascontiguousarray(plane_z, dtype=np.float32)
-        self._check(self.lib.kv_shift_and_add(self.handle, _ptr(proj), _host_ptr(sy), len(pz), _host_ptr(pz),
-                                              sid, fulcrum_z, voxel_dy, _ptr(out)))
+        self._dev(self.lib.kv_shift_and_add, _ptr(proj), _host_ptr(sy), len(pz), _host_ptr(pz), sid, fulcrum_z,
+                  voxel_dy, _ptr(out))
 
     def log_likelihood(self, data, vol) -> float:
         out = ctypes.c_double()
-        self._check(self.lib.kv_log_likelihood(self.handle, _ptr(data), _ptr(vol), ctypes.byref(out)))
+        self._dev(self.lib.kv_log_likelihood, _ptr(data), _ptr(vol), ctypes.byref(out))
         return out.value
 
     def trace_rays(self, rays, cap: int = 1024, mode: int = MODE_R):

@@ -211,7 +211,7 @@ kv_status kv_set_stream(kv_context *ctx, void *hip_stream)
 {
     if (kv_status st = check_ctx(ctx, false))
         return st;
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    ctx->stream = (hipStream_t)hip_stream;  // NULL selects the null stream
     return KV_OK;
 }
 

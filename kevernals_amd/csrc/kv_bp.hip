@@ -246,9 +246,11 @@ __global__ __launch_bounds__(256) void bp_r_kernel(KvGeom g, KvTables t, const f
 }
 
 // ------------------------------------------------------------ mode A -----
-template <int EP>
+// UNIT: backproject a projection of ones (the sensitivity A^T 1) without
+// reading `proj` at all.
+template <int EP, bool UNIT>
 __global__ __launch_bounds__(256) void bp_a_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
-                                                   float *__restrict__ vol, const float *__restrict__ sens, int unit)
+                                                   float *__restrict__ vol, const float *__restrict__ sens)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -282,13 +284,13 @@ __global__ __launch_bounds__(256) void bp_a_kernel(KvGeom g, KvTables t, const f
         const float *tx = t.x + (size_t)p * g.nu * nx1 + x;
         const float *lxr = t.lx2 + (size_t)p * g.nu;
         const float *lyr = t.ly2 + (size_t)p * g.nv;
-        const float *pr = proj + (size_t)p * g.nu * g.nv;
+        const size_t pbase = (size_t)p * g.nu * g.nv;
         for (int py = ly; py <= hy; ++py) {
             const float a1y = ty[(size_t)py * ny1], a2y = ty[(size_t)py * ny1 + 1];
             const float lo = fmaxf(fmaxf(0.f, fminf(a1y, a2y)), mnz);
             const float up = fminf(fminf(1.f, fmaxf(a1y, a2y)), mxz);
             const float ly2 = lyr[py];
-            const float *prow = pr + (size_t)py * g.nu;
+            const size_t rbase = pbase + (size_t)py * g.nu;
             for (int px = lx; px <= hx; ++px) {
                 const float a1x = tx[(size_t)px * nx1], a2x = tx[(size_t)px * nx1 + 1];
                 const float len = fminf(up, fmaxf(a1x, a2x)) - fmaxf(lo, fminf(a1x, a2x));
@@ -297,8 +299,10 @@ __global__ __launch_bounds__(256) void bp_a_kernel(KvGeom g, KvTables t, const f
                         acc += 1.f;
                     } else {
                         const float L = sqrtf((lxr[px] + ly2) + lz2);
-                        const float yv = unit ? 1.f : prow[px];
-                        acc += len * (L * yv);
+                        if (UNIT)
+                            acc += len * L;  // len * (L * 1)
+                        else
+                            acc += len * (L * proj[rbase + px]);
                     }
                 }
             }
@@ -344,11 +348,13 @@ hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj,
                           const float *sens, bool unit_input, hipStream_t s)
 {
     const dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), block(256);
-    if (epilogue == KV_EP_PLAIN)
-        bp_a_kernel<KV_EP_PLAIN><<<grid, block, 0, s>>>(g, t, proj, vol, sens, unit_input ? 1 : 0);
-    else if (epilogue == KV_EP_COUNT)
-        bp_a_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, t, proj, vol, sens, 1);
+    if (epilogue == KV_EP_COUNT)
+        bp_a_kernel<KV_EP_COUNT, true><<<grid, block, 0, s>>>(g, t, nullptr, vol, nullptr);
+    else if (epilogue == KV_EP_PLAIN && unit_input)
+        bp_a_kernel<KV_EP_PLAIN, true><<<grid, block, 0, s>>>(g, t, nullptr, vol, nullptr);
+    else if (epilogue == KV_EP_PLAIN)
+        bp_a_kernel<KV_EP_PLAIN, false><<<grid, block, 0, s>>>(g, t, proj, vol, nullptr);
     else
-        bp_a_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, t, proj, vol, sens, unit_input ? 1 : 0);
+        bp_a_kernel<KV_EP_MLEM, false><<<grid, block, 0, s>>>(g, t, proj, vol, sens);
     return hipGetLastError();
 }
