@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["full", "small"], default="full")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run an elementwise copy of 256 MiB (known bytes) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
@@ -135,6 +137,16 @@ def main():
     mu, data = make_data(K, ctx, g, torch)
     fp_pairs, bp_pairs = ctx.count_pairs(K.MODE_R)
     log(f"[bench] accepted pairs: forward {fp_pairs}, backprojector {bp_pairs}")
+
+    if args.copy_calibration:
+        # reads 256 MiB and writes 256 MiB: the FETCH_SIZE / WRITE_SIZE of this
+        # kernel calibrate the counters against known bytes
+        a = torch.rand(64 << 20, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     x = torch.ones(g.vol_shape, device="cuda")
     for _ in range(args.warmup):

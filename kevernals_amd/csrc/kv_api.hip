@@ -243,9 +243,10 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
         kv_set_error("voxel spacing and pixel pitch must be finite and positive");
         return KV_ERR_INVALID;
     }
-    // voxel indices are 32-bit unsigned in the kernels; ray ids 31-bit
+    // voxel gathers use 32-bit byte offsets (at most 2^30 voxels); ray ids
+    // are 31-bit
     const int64_t nvox = (int64_t)q.nx * q.ny * q.nz, nray = (int64_t)q.np * q.nu * q.nv;
-    if (nvox > 0xFFFFFFFFLL || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24)) {
+    if (nvox > (1LL << 30) || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24)) {
         kv_set_error("problem too large: %lld voxels, %lld rays", (long long)nvox, (long long)nray);
         return KV_ERR_INVALID;
     }

@@ -146,6 +146,17 @@ constexpr int YBR = 96;          // y band rows
 
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
+// min/max as signed-integer min/max of the IEEE bit patterns.  They order two
+// floats exactly as fminf/fmaxf when both are >= +0, and put every negative
+// value (including -0) below every non-negative one.  The fast path uses them
+// only in w = min(H, Hy) - max(L, Ly) with L >= +0 (L carries the clip at
+// alpha = 0 as +0): w > 0 needs all of H, Hy and min(H, Hy) positive, where
+// the results are the FP32 ones bit for bit; when any of them is negative both
+// forms give w < 0.  No NaN reaches these operands (entries are finite or
+// +-INF).
+__device__ __forceinline__ float bmin(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
+__device__ __forceinline__ float bmax(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
+
 // Detector index range [lo, hi] (pixel-centre units) whose centres fall in
 // the shadow of [c, c+1] (lateral, floating voxels) between depth factors
 // t0 and t1, widened by `pad` pixels.  Returns false when the shadow is not
@@ -174,8 +185,11 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
     __shared__ int xbase[TX + 1];
     __shared__ float yb[(TY + 1) * YBR];
     __shared__ int ybase_s;
+    // per-voxel running sums (value, weight), one pair per slice of the brick
+    __shared__ float accs[ZT][2][256];
 
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * ZT;
     const int x = x0 + lane, y = y0 + wv;
     const int nzt = min(ZT, g.nz - z0);
@@ -188,10 +202,8 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
     const int half = 5;
     const float wx = (float)half * g.pu / g.sx, wy = (float)half * g.pv / g.sy;
 
-    float acc[ZT], wsum[ZT];
-#pragma unroll
     for (int i = 0; i < ZT; ++i)
-        acc[i] = wsum[i] = 0.f;
+        accs[i][0][tid] = accs[i][1][tid] = 0.f;
 
     for (int p = 0; p < g.np; ++p) {
         const float *S = g.src + 3 * p;
@@ -205,9 +217,9 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
         const float *ty = t.y + (size_t)p * ny1 * g.nv;
 
         __syncthreads();  // the previous view's readers are done with the bands
-        if (threadIdx.x <= TX) {
+        if (tid <= TX) {
             // band of plane k: shadows of planes k-1 and k+1 at both depths
-            const int k = x0 + threadIdx.x;
+            const int k = x0 + tid;
             float mn = kInf;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
@@ -215,8 +227,8 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
                 mn = fminf(mn, fminf(c * ta, c * tb));
             }
             const float u = clampf((sfx + mn - ofx) * scx - 0.5f, -1e6f, 1e6f);
-            xbase[threadIdx.x] = (int)floorf(u) - 2;
-        } else if (threadIdx.x == TX + 1) {
+            xbase[tid] = (int)floorf(u) - 2;
+        } else if (tid == TX + 1) {
             float mn = kInf;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
@@ -227,7 +239,7 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
             ybase_s = (int)floorf(v) - 2;
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < (TX + 1) * XBW; e += 256) {
+        for (int e = tid; e < (TX + 1) * XBW; e += 256) {
             const int i = e / XBW, j = e - i * XBW;
             const int k = x0 + i, px = xbase[i] + j;
             float v = 0.f;
@@ -236,7 +248,7 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
             xb[i * XBS + j] = v;
         }
         const int ybase = ybase_s;
-        for (int e = threadIdx.x; e < (TY + 1) * YBR; e += 256) {
+        for (int e = tid; e < (TY + 1) * YBR; e += 256) {
             const int c = e / YBR, r = e - c * YBR;
             const int j = y0 + c, py = ybase + r;
             float v = 0.f;
@@ -256,117 +268,168 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
         const float *lx2 = MODE == KV_MODE_A ? t.lx2 + (size_t)p * g.nu : nullptr;
         const float *ly2 = MODE == KV_MODE_A ? t.ly2 + (size_t)p * g.nv : nullptr;
         const float lz2 = MODE == KV_MODE_A ? t.lz2[p] : 0.f;
+        const float *xr0 = xb + lane * XBS, *xr1 = xb + (lane + 1) * XBS;
+        const float *yr0 = yb + wv * YBR, *yr1 = yb + (wv + 1) * YBR;
 
+        for (int zi = 0; zi < nzt; ++zi) {
+            const int z = z0 + zi;
+            const float cvz = (float)z - hz;
+            int lx = 0, hxw = g.nu - 1, ly = 0, hyw = g.nv - 1;
+            if (MODE == KV_MODE_R) {
+                // voxel centre on the detector and the +-5 window,
+                // Projector.cu:473-492, 607-646
+                const float rdz = (cvz + 0.5f) - sfz;
+                const float az = fabsf(rdz) > 1e-24f ? (ofz - sfz) / rdz : 0.f;
+                const float ix = sfx + rdx * az;
+                const float iy = sfy + rdy * az;
+                if (!(ix > ofx - wx && ix < efx + wx && iy > ofy - wy && iy < efy + wy))
+                    continue;
+                const int pcx = (int)((ix - ofx) * g.sx / g.pu);
+                const int pcy = (int)((iy - ofy) * g.sy / g.pv);
+                lx = pcx - half;
+                hxw = pcx + half;
+                ly = pcy - half;
+                hyw = pcy + half;
+                lx = lx < 0 ? 0 : (lx >= g.nu ? g.nu - 1 : lx);
+                hxw = hxw < 0 ? 0 : (hxw >= g.nu ? g.nu - 1 : hxw);
+                ly = ly < 0 ? 0 : (ly >= g.nv ? g.nv - 1 : ly);
+                hyw = hyw < 0 ? 0 : (hyw >= g.nv ? g.nv - 1 : hyw);
+            }
+            // shadow of the voxel box
+            const float t0 = dd / (cvz - sfz), t1 = dd / ((cvz + 1.f) - sfz);
+            int fx0, fx1, fy0, fy1;
+            if (shadow(cvx, sfx, ofx, t0, t1, scx, g.nu, 0.01f, fx0, fx1) &&
+                shadow(cvy, sfy, ofy, t0, t1, scy, g.nv, 0.01f, fy0, fy1)) {
+                lx = max(lx, fx0);
+                hxw = min(hxw, fx1);
+                ly = max(ly, fy0);
+                hyw = min(hyw, fy1);
+            }
+            if (lx > hxw || ly > hyw)
+                continue;
+            const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
+            const float lz = fmaxf(0.f, fminf(a1z, a2z)), uz = fminf(1.f, fmaxf(a1z, a2z));
+            // one running sum per voxel across all views, as the reference
+            // accumulates (Projector.cu:727-728)
+            float a = accs[zi][0][tid], wsm = accs[zi][1][tid];
+            const float lzp = bmax(0.f, fminf(a1z, a2z));  // lz with a +0 floor in bit order
+            const int ncol = hxw - lx + 1;
+            const bool in_band = lx - xb0 >= 0 && hxw - xb0 < XBW && lx - xb1 >= 0 && hxw - xb1 < XBW;
+            const bool rows_in_band = ly - ybase >= 0 && hyw - ybase < YBR;  // wave-uniform
+            if (ncol <= 4 && in_band && rows_in_band) {
+                // x intervals of the (at most four) columns stay in registers
+                float Lx[4], Hx[4], L2[4];
+                int cx[4];
 #pragma unroll
-        for (int zi = 0; zi < ZT; ++zi) {
-            if (zi < nzt) {
-                const int z = z0 + zi;
-                const float cvz = (float)z - hz;
-                int lx = 0, hxw = g.nu - 1, ly = 0, hyw = g.nv - 1;
-                bool take = true;
-                if (MODE == KV_MODE_R) {
-                    // voxel centre on the detector and the +-5 window,
-                    // Projector.cu:473-492, 607-646
-                    const float rdz = (cvz + 0.5f) - sfz;
-                    const float az = fabsf(rdz) > 1e-24f ? (ofz - sfz) / rdz : 0.f;
-                    const float ix = sfx + rdx * az;
-                    const float iy = sfy + rdy * az;
-                    take = ix > ofx - wx && ix < efx + wx && iy > ofy - wy && iy < efy + wy;
-                    if (take) {
-                        const int pcx = (int)((ix - ofx) * g.sx / g.pu);
-                        const int pcy = (int)((iy - ofy) * g.sy / g.pv);
-                        lx = pcx - half;
-                        hxw = pcx + half;
-                        ly = pcy - half;
-                        hyw = pcy + half;
-                        lx = lx < 0 ? 0 : (lx >= g.nu ? g.nu - 1 : lx);
-                        hxw = hxw < 0 ? 0 : (hxw >= g.nu ? g.nu - 1 : hxw);
-                        ly = ly < 0 ? 0 : (ly >= g.nv ? g.nv - 1 : ly);
-                        hyw = hyw < 0 ? 0 : (hyw >= g.nv ? g.nv - 1 : hyw);
+                for (int c = 0; c < 4; ++c) {
+                    Lx[c] = 1.f;
+                    Hx[c] = 0.f;  // empty: any weight <= -1
+                    L2[c] = 0.f;
+                    cx[c] = min(lx + c, g.nu - 1);  // in-bounds address for every slot
+                    if (c < ncol) {
+                        const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
+                        Lx[c] = bmax(lzp, bmin(a1x, a2x));
+                        Hx[c] = bmin(uz, bmax(a1x, a2x));
+                        if (MODE == KV_MODE_A)
+                            L2[c] = lx2[lx + c];
                     }
                 }
-                if (take) {
-                    // shadow of the voxel box
-                    const float t0 = dd / (cvz - sfz), t1 = dd / ((cvz + 1.f) - sfz);
-                    int fx0, fx1, fy0, fy1;
-                    if (shadow(cvx, sfx, ofx, t0, t1, scx, g.nu, 0.01f, fx0, fx1) &&
-                        shadow(cvy, sfy, ofy, t0, t1, scy, g.nv, 0.01f, fy0, fy1)) {
-                        lx = max(lx, fx0);
-                        hxw = min(hxw, fx1);
-                        ly = max(ly, fy0);
-                        hyw = min(hyw, fy1);
+                for (int py = ly; py <= hyw; ++py) {
+                    const int r = py - ybase;
+                    const float a1y = yr0[r], a2y = yr1[r];
+                    // max/min are exact: the reference's grouping
+                    // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
+                    const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
+                    const float *prow = pr + (size_t)py * g.nu;
+                    const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
+                    float v[4];
+                    if (EP != KV_EP_COUNT && !UNIT) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            v[c] = prow[cx[c]];  // issued together; used only if accepted
                     }
-                    const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
-                    const float lz = fmaxf(0.f, fminf(a1z, a2z)), uz = fminf(1.f, fmaxf(a1z, a2z));
-                    // one running sum per voxel across all views, as the
-                    // reference accumulates (Projector.cu:727-728)
-                    float &a = acc[zi];
-                    float &wsm = wsum[zi];
-                    for (int py = ly; py <= hyw; ++py) {
-                        const int r = py - ybase;
-                        float a1y, a2y;
-                        if (r >= 0 && r < YBR) {
-                            a1y = yb[wv * YBR + r];
-                            a2y = yb[(wv + 1) * YBR + r];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
+                        const bool ok = w > tol;
+                        // selects, not adds of zero: skipped pixels leave the
+                        // running sums bit-identical
+                        if (EP == KV_EP_COUNT) {
+                            wsm = ok ? wsm + 1.f : wsm;
+                        } else if (MODE == KV_MODE_R) {
+                            wsm = ok ? wsm + w : wsm;
+                            a = ok ? a + w * v[c] : a;
                         } else {
-                            a1y = ty[(size_t)y * g.nv + py];
-                            a2y = ty[(size_t)(y + 1) * g.nv + py];
+                            const float L = sqrtf((L2[c] + lyy) + lz2);
+                            if (UNIT)
+                                a = ok ? a + w * L : a;
+                            else
+                                a = ok ? a + w * (L * v[c]) : a;
                         }
-                        // max/min are exact: the reference's grouping
-                        // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
-                        const float lo = fmaxf(lz, fminf(a1y, a2y));
-                        const float up = fminf(uz, fmaxf(a1y, a2y));
-                        const float *prow = pr + (size_t)py * g.nu;
-                        const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
-                        for (int px = lx; px <= hxw; ++px) {
-                            const int j0 = px - xb0, j1 = px - xb1;
-                            const float a1x = (j0 >= 0 && j0 < XBW) ? xb[lane * XBS + j0] : tx[(size_t)x * g.nu + px];
-                            const float a2x =
-                                (j1 >= 0 && j1 < XBW) ? xb[(lane + 1) * XBS + j1] : tx[(size_t)(x + 1) * g.nu + px];
-                            const float w = fminf(up, fmaxf(a1x, a2x)) - fmaxf(lo, fminf(a1x, a2x));
-                            if (w > tol) {
-                                if (EP == KV_EP_COUNT) {
-                                    wsm += 1.f;
-                                } else if (MODE == KV_MODE_R) {
-                                    wsm += w;
-                                    a += w * prow[px];
-                                } else {
-                                    const float L = sqrtf((lx2[px] + lyy) + lz2);
-                                    if (UNIT)
-                                        a += w * L;
-                                    else
-                                        a += w * (L * prow[px]);
-                                }
+                    }
+                }
+            } else {
+                for (int py = ly; py <= hyw; ++py) {
+                    const int r = py - ybase;
+                    float a1y, a2y;
+                    if (r >= 0 && r < YBR) {
+                        a1y = yr0[r];
+                        a2y = yr1[r];
+                    } else {
+                        a1y = ty[(size_t)y * g.nv + py];
+                        a2y = ty[(size_t)(y + 1) * g.nv + py];
+                    }
+                    const float lo = fmaxf(lz, fminf(a1y, a2y));
+                    const float up = fminf(uz, fmaxf(a1y, a2y));
+                    const float *prow = pr + (size_t)py * g.nu;
+                    const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
+                    for (int px = lx; px <= hxw; ++px) {
+                        const int j0 = px - xb0, j1 = px - xb1;
+                        const float a1x = (j0 >= 0 && j0 < XBW) ? xr0[j0] : tx[(size_t)x * g.nu + px];
+                        const float a2x = (j1 >= 0 && j1 < XBW) ? xr1[j1] : tx[(size_t)(x + 1) * g.nu + px];
+                        const float w = fminf(up, fmaxf(a1x, a2x)) - fmaxf(lo, fminf(a1x, a2x));
+                        if (w > tol) {
+                            if (EP == KV_EP_COUNT) {
+                                wsm += 1.f;
+                            } else if (MODE == KV_MODE_R) {
+                                wsm += w;
+                                a += w * prow[px];
+                            } else {
+                                const float L = sqrtf((lx2[px] + lyy) + lz2);
+                                if (UNIT)
+                                    a += w * L;
+                                else
+                                    a += w * (L * prow[px]);
                             }
                         }
                     }
                 }
             }
+            accs[zi][0][tid] = a;
+            accs[zi][1][tid] = wsm;
         }
     }
     if (!active)
         return;
-#pragma unroll
-    for (int zi = 0; zi < ZT; ++zi) {
-        if (zi < nzt) {
-            const size_t o = ((size_t)(z0 + zi) * g.ny + y) * g.nx + x;
-            if (EP == KV_EP_COUNT) {
-                vol[o] = wsum[zi];
-            } else if (MODE == KV_MODE_R) {
-                const float bp = wsum[zi] > 1e-24f ? acc[zi] / wsum[zi] : 0.f;  // Projector.cu:773-776
-                if (EP == KV_EP_PLAIN)
-                    vol[o] = bp;
-                else if (EP == KV_EP_MLEM)
-                    vol[o] = vol[o] * bp;  // Reconstructors.h:238-241
-                else
-                    vol[o] = vol[o] + bp;  // Reconstructors.h:118-121
+    for (int zi = 0; zi < nzt; ++zi) {
+        const size_t o = ((size_t)(z0 + zi) * g.ny + y) * g.nx + x;
+        const float a = accs[zi][0][tid], wsm = accs[zi][1][tid];
+        if (EP == KV_EP_COUNT) {
+            vol[o] = wsm;
+        } else if (MODE == KV_MODE_R) {
+            const float bp = wsm > 1e-24f ? a / wsm : 0.f;  // Projector.cu:773-776
+            if (EP == KV_EP_PLAIN)
+                vol[o] = bp;
+            else if (EP == KV_EP_MLEM)
+                vol[o] = vol[o] * bp;  // Reconstructors.h:238-241
+            else
+                vol[o] = vol[o] + bp;  // Reconstructors.h:118-121
+        } else {
+            if (EP == KV_EP_PLAIN) {
+                vol[o] = a;
             } else {
-                if (EP == KV_EP_PLAIN) {
-                    vol[o] = acc[zi];
-                } else {
-                    const float s = sens[o];
-                    vol[o] = s > 0.f ? (vol[o] * acc[zi]) / s : vol[o];
-                }
+                const float s = sens[o];
+                vol[o] = s > 0.f ? (vol[o] * a) / s : vol[o];
             }
         }
     }

@@ -125,46 +125,59 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     }
     int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
 
+    // A run's head holds +INF once the run is exhausted, so the next list
+    // entry is the plain minimum of the four heads.  Equal heads are
+    // interchangeable: the sorted sequence of values is unique.
+    if (cx == 0) ax = kInf;
+    if (cy == 0) ay = kInf;
+    if (cz == 0) az = kInf;
+    float af = cf > 0 ? 1.0f : kInf;
     auto pop = [&]() -> float {
-        const float vx = cx > 0 ? ax : kInf;
-        const float vy = cy > 0 ? ay : kInf;
-        const float vz = cz > 0 ? az : kInf;
-        const float vf = cf > 0 ? 1.0f : kInf;
-        const float m = fminf(fminf(vx, vy), fminf(vz, vf));
-        const bool tx = vx == m;
-        const bool ty = !tx && vy == m;
-        const bool tz = !tx && !ty && vz == m;
+        const float m = fminf(fminf(ax, ay), fminf(az, af));
+        const bool tx = ax == m;
+        const bool ty = !tx && ay == m;
+        const bool tz = !tx && !ty && az == m;
         const bool tf = !tx && !ty && !tz;
-        ax = tx ? ax + ix : ax;
-        ay = ty ? ay + iy : ay;
-        az = tz ? az + iz : az;
         cx -= tx;
         cy -= ty;
         cz -= tz;
         cf -= tf;
+        // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
+        ax = tx ? (cx == 0 ? kInf : ax + ix) : ax;
+        ay = ty ? (cy == 0 ? kInf : ay + iy) : ay;
+        az = tz ? (cz == 0 ? kInf : az + iz) : az;
+        af = cf == 0 ? kInf : af;
         return m;
     };
 
     // Segment loop, Projector.cu:348-367.  The midpoint is the FP32 sum
-    // divided by 2.0 in FP64; (double)float * (double)float is exact, so the
-    // index expression equals one FP64 fma.
+    // divided by 2.0 in FP64.  (double)float * (double)float is exact, and
+    // so is the halving, so each index expression equals one FP64 fma of
+    // the FP32 sum with the halved direction.
     const double bx = (double)(r.hx + r.sx);
     const double by = (double)(r.hy + r.sy);
     const double bz = (double)(r.hz + r.sz);
-    const double ddx = (double)r.dx, ddy = (double)r.dy, ddz = (double)r.dz;
+    const double hdx = 0.5 * (double)r.dx, hdy = 0.5 * (double)r.dy, hdz = 0.5 * (double)r.dz;
     const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz;
     float prev = pop();
     for (int i = 0; i < n - 1; ++i) {
         const float cur = pop();
         const float w = cur - prev;
-        const double mid = (double)(cur + prev) * 0.5;
-        const int vx = (int)__builtin_fma(mid, ddx, bx);
-        const int vy = (int)__builtin_fma(mid, ddy, by);
-        const int vz = (int)__builtin_fma(mid, ddz, bz);
+        const double sum = (double)(cur + prev);
+        const int vx = (int)__builtin_fma(sum, hdx, bx);
+        const int vy = (int)__builtin_fma(sum, hdy, by);
+        const int vz = (int)__builtin_fma(sum, hdz, bz);
         if ((unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol)
             visit(((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx, w);
         prev = cur;
     }
+}
+
+// Voxel gather with a 32-bit byte offset from the (wave-uniform) volume base:
+// volumes are limited to 2^30 voxels, so k << 2 cannot wrap.
+__device__ __forceinline__ float ld_voxel(const float *__restrict__ base, unsigned k)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (k << 2));
 }
 
 // AlmostEqualRelative(v, 0) of commons/Maths.h:25-34 in FP32.
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__rest
     float tot = 0.f, tw = 0.f;
     walk_r(g, r, [&](unsigned k, float w) {
         tw += w;
-        tot += w * vol[k];
+        tot += w * ld_voxel(vol, k);
     });
     const float fp = tw > 1e-6f ? tot / tw : 0.f;
     if (EP == KV_EP_PLAIN) {
@@ -357,7 +370,7 @@ __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__rest
         return;
     }
     float acc = 0.f;
-    walk_a(r, [&](unsigned k, float len) { acc += len * vol[k]; });
+    walk_a(r, [&](unsigned k, float len) { acc += len * ld_voxel(vol, k); });
     const float ax = r.L * acc;
     if (EP == KV_EP_PLAIN)
         out[o] = ax;
