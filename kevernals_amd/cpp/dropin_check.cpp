@@ -1,0 +1,135 @@
+// Runs the drop-in Projector and recons:: the way the reference's callers do
+// (mainReconstruct.cpp:170-219) on inputs read from a directory, and writes
+// the results as raw float32 files for comparison.
+//
+//   kv_dropin_check <dir>
+//   <dir>/geom.txt : nx ny nz sx sy sz nu nv np pu pv fulcrum_z, then per view
+//                    src_x src_y src_z det_x det_y det_z
+//   <dir>/vol.bin, <dir>/proj.bin : float32 volume and projections
+// Writes fp.bin, bp.bin, mlem.bin, mlem_default.bin, art.bin, saa.bin and
+// status.txt (one "name code" line per call).
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "Reconstructors.h"
+
+namespace {
+bool read_raw(const std::string &path, float *dst, size_t n)
+{
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f)
+        return false;
+    const size_t got = std::fread(dst, sizeof(float), n, f);
+    std::fclose(f);
+    return got == n;
+}
+
+void write_raw(const std::string &path, ImageDataPtr img)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f)
+        return;
+    int d[3];
+    img->GetDimensions(d);
+    std::fwrite(img->GetScalarPointer(), sizeof(float), (size_t)d[0] * d[1] * d[2], f);
+    std::fclose(f);
+}
+
+ImageDataPtr image(int x, int y, int z, double sx, double sy, double sz)
+{
+    auto img = ImageDataPtr::New();
+    img->SetDimensions(x, y, z);
+    img->SetSpacing(sx, sy, sz);
+    img->AllocateScalars(VTK_FLOAT, 1);
+    return img;
+}
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::cerr << "usage: kv_dropin_check <dir>" << std::endl;
+        return 2;
+    }
+    const std::string dir = argv[1];
+    std::ifstream in(dir + "/geom.txt");
+    int nx, ny, nz, nu, nv, np;
+    float sx, sy, sz, pu, pv, fz;
+    if (!(in >> nx >> ny >> nz >> sx >> sy >> sz >> nu >> nv >> np >> pu >> pv >> fz)) {
+        std::cerr << "bad geom.txt" << std::endl;
+        return 2;
+    }
+    std::vector<Position3D> src((size_t)np);
+    std::vector<Position2D> bl((size_t)np);
+    std::vector<float> dz((size_t)np);
+    for (int p = 0; p < np; ++p) {
+        float a, b, c, d, e, f;
+        in >> a >> b >> c >> d >> e >> f;
+        src[(size_t)p] = Position3D(a, b, c);
+        bl[(size_t)p] = Position2D(d, e);
+        dz[(size_t)p] = f;
+    }
+    TomoGeometry geom(Size3D(nx, ny, nz), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl, dz,
+                      WSize3D(0.f, 0.f, fz));
+    auto vol = image(nx, ny, nz, sx, sy, sz);
+    auto proj = image(nu, nv, np, pu, pv, 1.0);
+    if (!read_raw(dir + "/vol.bin", static_cast<float *>(vol->GetScalarPointer()), (size_t)nx * ny * nz) ||
+        !read_raw(dir + "/proj.bin", static_cast<float *>(proj->GetScalarPointer()), (size_t)nu * nv * np)) {
+        std::cerr << "cannot read inputs" << std::endl;
+        return 2;
+    }
+    std::ofstream status(dir + "/status.txt");
+    int failures = 0;
+    auto note = [&](const char *name, std::error_code ec) {
+        status << name << " " << ec.value() << std::endl;
+        if (ec) {
+            ++failures;
+            std::cerr << name << ": " << ec.message() << std::endl;
+        }
+    };
+
+    Projector projector(&geom);
+    auto fp = projector.PerformProjection(vol);
+    note("projection", fp == nullptr ? projector.lastError() : std::error_code());
+    if (fp != nullptr)
+        write_raw(dir + "/fp.bin", fp);
+    auto bp = projector.PerformBackProjection(proj);
+    note("backprojection", bp == nullptr ? projector.lastError() : std::error_code());
+    if (bp != nullptr)
+        write_raw(dir + "/bp.bin", bp);
+
+    // copies of the volume: ART/MLEM mutate their initial volume (Q21)
+    auto copy = [&](ImageDataPtr src_img) {
+        auto c = image(nx, ny, nz, sx, sy, sz);
+        std::copy((float *)src_img->GetScalarPointer(), (float *)src_img->GetScalarPointer() + (size_t)nx * ny * nz,
+                  (float *)c->GetScalarPointer());
+        return c;
+    };
+    auto mlem = recons::MLEM(&geom, proj, 3, 1.0f, copy(vol), std::nullopt);
+    note("mlem", mlem.has_value() ? std::error_code() : mlem.error());
+    if (mlem)
+        write_raw(dir + "/mlem.bin", mlem.value());
+    auto mlem0 = recons::MLEM(&geom, proj, 2, 1.0f, std::nullopt, std::nullopt);
+    note("mlem_default", mlem0.has_value() ? std::error_code() : mlem0.error());
+    if (mlem0)
+        write_raw(dir + "/mlem_default.bin", mlem0.value());
+    auto art = recons::ART(&geom, proj, 2, 0.5f, copy(vol), std::nullopt);
+    note("art", art.has_value() ? std::error_code() : art.error());
+    if (art)
+        write_raw(dir + "/art.bin", art.value());
+    auto bpr = recons::BackProjection(&geom, proj);
+    note("recons_backprojection", bpr.has_value() ? std::error_code() : bpr.error());
+    auto saa = recons::ShiftAndAdd(&geom, proj, std::nullopt);
+    note("saa", saa.has_value() ? std::error_code() : saa.error());
+    if (saa)
+        write_raw(dir + "/saa.bin", saa.value());
+
+    // error path: one reconstructed slice (Reconstructors.h:274-279)
+    TomoGeometry flat(Size3D(nx, ny, 1), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl, dz);
+    auto bad = recons::ShiftAndAdd(&flat, proj, std::nullopt);
+    status << "saa_one_slice " << (bad.has_value() ? 0 : bad.error().value()) << std::endl;
+    return failures == 0 ? 0 : 1;
+}

@@ -1,0 +1,73 @@
+"""The C++ drop-in (Projector / recons:: over the C ABI) driven as the
+reference's callers drive it, against the oracle on the geometry the drop-in
+marshals: source x and z averaged over views in FP32 (TomoTable.cpp:11-18),
+common detector height (TomoGeometry.cpp:345), per-view source y and detector
+corner (Projector.cu:917-936)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHECK = os.path.join(ROOT, "kevernals_amd", "lib", "kv_dropin_check")
+
+
+def marshalled(g):
+    """The kernel geometry the drop-in builds from TomoGeometry accessors."""
+    from kevernals_amd.geometry import Geometry
+
+    sx = np.float32(0.0)
+    sz = np.float32(0.0)
+    for p in range(g.nviews):
+        sx = np.float32(sx + g.src[p, 0])
+        sz = np.float32(sz + g.src[p, 2])
+    sx = np.float32(sx / np.float32(g.nviews))
+    sz = np.float32(sz / np.float32(g.nviews))
+    dz = np.float32(np.float32(sum(float(v) for v in g.det[:, 2])) / np.float32(g.nviews))
+    src = np.stack([np.full(g.nviews, sx), g.src[:, 1], np.full(g.nviews, sz)], axis=1).astype(np.float32)
+    det = np.stack([g.det[:, 0], g.det[:, 1], np.full(g.nviews, dz)], axis=1).astype(np.float32)
+    return Geometry(g.nx, g.ny, g.nz, g.sx, g.sy, g.sz, g.nu, g.nv, g.nviews, g.pu, g.pv, det, src), sz, dz
+
+
+def test_dropin_matches_oracle(tmp_path, oracle_mod):
+    from kevernals_amd import geometry as G
+    from kevernals_amd.phantom import phantom
+
+    assert os.path.exists(CHECK), "drop-in not built"
+    g = G.scaled_down(G.small_plain(), 2)
+    gm, sz, dz = marshalled(g)
+    o = oracle_mod.Oracle(gm)
+    vol = phantom(gm, 5)
+    proj = o.fp_r(phantom(gm, 6))
+    fz = 0.0
+    with open(tmp_path / "geom.txt", "w") as f:
+        f.write(f"{g.nx} {g.ny} {g.nz} {g.sx!r} {g.sy!r} {g.sz!r} {g.nu} {g.nv} {g.nviews} {g.pu!r} {g.pv!r} {fz}\n")
+        for p in range(g.nviews):
+            vals = [*g.src[p], *g.det[p]]
+            f.write(" ".join(f"{float(v):.9g}" for v in vals) + "\n")
+    vol.tofile(tmp_path / "vol.bin")
+    proj.tofile(tmp_path / "proj.bin")
+    r = subprocess.run([CHECK, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    status = dict(line.split() for line in open(tmp_path / "status.txt"))
+    assert all(v == "0" for k, v in status.items() if k != "saa_one_slice")
+    assert status["saa_one_slice"] == "5"  # ReconstructorsErrorCode::ShiftAndAdd
+
+    def rd(name, shape):
+        return np.fromfile(tmp_path / name, dtype=np.float32).reshape(shape)
+
+    np.testing.assert_array_equal(rd("fp.bin", gm.proj_shape), o.fp_r(vol))
+    np.testing.assert_array_equal(rd("bp.bin", gm.vol_shape), o.bp_r(proj))
+    np.testing.assert_array_equal(rd("mlem.bin", gm.vol_shape), o.mlem(proj, vol, 3, 1.0))
+    np.testing.assert_array_equal(rd("mlem_default.bin", gm.vol_shape),
+                                  o.mlem(proj, np.ones(gm.vol_shape, np.float32), 2, 1.0))
+    np.testing.assert_array_equal(rd("art.bin", gm.vol_shape), o.art(proj, vol, 2, 0.5))
+    planes = gm.plane_heights()
+    sid = np.float32(abs(np.float32(dz) - np.float32(sz)))
+    rc, saa = oracle_mod.saa(proj, gm.src[:, 1], planes, float(sid), fz, gm.sy)
+    assert rc == 0
+    np.testing.assert_array_equal(rd("saa.bin", (len(planes), gm.nv, gm.nu)), saa)
