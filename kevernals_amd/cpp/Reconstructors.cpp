@@ -7,6 +7,9 @@
 
 #include "kevernals_amd.h"
 
+#ifdef KV_USE_REFERENCE_TREE
+#include "modules/dataHandling/PhantomMaker.h"
+#else
 // ------------------------------------------------------------ error codes --
 namespace {
 class ReconstructorsCategory : public std::error_category {
@@ -46,6 +49,7 @@ const ProjectorCategory g_projector_category;
 
 std::error_code make_error_code(ReconstructorsErrorCode e) { return {(int)e, g_recons_category}; }
 std::error_code make_error_code(ProjectorErrorCode e) { return {(int)e, g_projector_category}; }
+#endif  // KV_USE_REFERENCE_TREE
 
 // ------------------------------------------------------------ marshalling --
 namespace {
@@ -260,6 +264,7 @@ vtkSmartPointer<vtkImageData> Projector::PerformBackProjection(vtkSmartPointer<v
     return out;
 }
 
+#ifndef KV_USE_REFERENCE_TREE
 ImageDataPtr PhantomMaker::GetPhantom() const
 {
     if (!m_volume)
@@ -271,6 +276,7 @@ ImageDataPtr PhantomMaker::GetPhantom() const
     std::fill(d, d + points(img), m_backgroundDensity);
     return img;
 }
+#endif  // KV_USE_REFERENCE_TREE
 
 // ----------------------------------------------------------------- recons --
 namespace recons {

@@ -13,15 +13,26 @@
 //  * the verbose TIFF dumps of p_outputDirectoryPath are not written.
 #pragma once
 
+//
+// Inside the reference source tree, define KV_USE_REFERENCE_TREE (and
+// KV_USE_REAL_VTK): the reference's own TomoGeometry, Result and error-code
+// headers are used and only Projector / recons:: come from this file.
 #include <optional>
 #include <string>
 #include <system_error>
 #include <utility>
 
-#include "TomoGeometry.h"
 #include "vtk_subset.h"
 
 struct kv_context;
+
+#ifdef KV_USE_REFERENCE_TREE
+#include "commons/Result.h"
+#include "modules/geometry/TomoGeometry.h"
+#include "modules/reconstruction/ProjectorErrorCode.h"
+#include "modules/reconstruction/ReconstructorsErrorCode.h"
+#else
+#include "TomoGeometry.h"
 
 // ---- error codes (ReconstructorsErrorCode.h:19-25, ProjectorErrorCode.h:19-27)
 enum class ReconstructorsErrorCode { ART = 1, BackProjectionFailed, ProjectionFailed, MLEM, ShiftAndAdd };
@@ -61,6 +72,7 @@ private:
     std::optional<T> value_;
     std::error_code error_;
 };
+#endif  // KV_USE_REFERENCE_TREE
 
 // ---- Projector (Projector.h:10-28)
 class Projector {
@@ -107,6 +119,7 @@ Result<ImageDataPtr> ShiftAndAdd(TomoGeometry *p_tomoGeometry, ImageDataPtr p_pr
                                  std::optional<std::string> p_outputDirectoryPath);
 }  // namespace recons
 
+#ifndef KV_USE_REFERENCE_TREE
 // ---- PhantomMaker subset (PhantomMaker.h:39-70): the uniform volume used as
 // the default initial volume of ART/MLEM (Reconstructors.h:35-36,152-153).
 class PhantomMaker {
@@ -121,3 +134,4 @@ private:
     TomoVolume *m_volume;
     float m_backgroundDensity;
 };
+#endif  // KV_USE_REFERENCE_TREE
