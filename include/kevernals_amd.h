@@ -147,6 +147,37 @@ kv_status kv_trace_rays(kv_context *ctx, int mode, int nrays, const int32_t *ray
  * These give the algorithmic bytes of a pass.  Synchronous. */
 kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *bp_pairs);
 
+/* ---- batched reconstructions (no reference counterpart) ------------------
+ * Many reconstructions of uneven volume and projection-count sizes share one
+ * persistent forward launch and one persistent backprojection launch per
+ * iteration (mode R, fused MLEM: the per-reconstruction result is bit-identical
+ * to kv_mlem_iteration on its own); the iterations can be captured once into
+ * a hipGraph and replayed.  The batch owns its device buffers and stream. */
+typedef struct kv_batch kv_batch;
+kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch **out);
+kv_status kv_batch_destroy(kv_batch *b);
+/* Host <-> device copies of reconstruction i's volume (initial estimate in,
+ * result out) and data (projections in). */
+kv_status kv_batch_set_volume(kv_batch *b, int i, const float *h_vol);
+kv_status kv_batch_set_data(kv_batch *b, int i, const float *h_data);
+kv_status kv_batch_get_volume(kv_batch *b, int i, float *h_vol);
+/* iters[i] MLEM iterations for reconstruction i (0 = untouched), relaxation
+ * `relax` as recons::MLEM.  use_graph != 0 captures the iteration sequence in
+ * a hipGraph (re-captured only when the counts or tile shapes change) and
+ * replays it.  *ms (optional) receives the device time of the run.
+ * Synchronous. */
+kv_status kv_batch_mlem(kv_batch *b, const int32_t *iters, float relax, int use_graph, float *ms);
+/* Tile shapes of reconstruction i: forward layout 1, 2 or 4 (detector rows
+ * per wavefront), backprojection brick depth 4 or 8 slices.  Results do not
+ * depend on them. */
+kv_status kv_batch_set_tiles(kv_batch *b, int i, int fp_layout, int bp_slices);
+kv_status kv_batch_get_tiles(kv_batch *b, int i, int *fp_layout, int *bp_slices);
+/* Times the tile shapes per size class (volume voxel count < 16 Mi, < 40 Mi,
+ * larger) on this batch and applies the fastest; with a cache file, classes
+ * already tuned on this device are read from it instead and new results are
+ * written back.  Volumes are restored afterwards. */
+kv_status kv_batch_autotune(kv_batch *b, const char *cache_path);
+
 /* ---- host-buffer variants (synchronous; used by the C++ Projector) ------- */
 kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode);
 kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode);

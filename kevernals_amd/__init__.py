@@ -81,6 +81,15 @@ _SIGS = {
     "kv_trace_rays": (_I, [_P, _I, _I, _P, _I, _P, _P, _P]),
     "kv_forward_project_host": (_I, [_P, _P, _P, _I]),
     "kv_back_project_host": (_I, [_P, _P, _P, _I]),
+    "kv_batch_create": (_I, [_I, _I, _P, ctypes.POINTER(_P)]),
+    "kv_batch_destroy": (_I, [_P]),
+    "kv_batch_set_volume": (_I, [_P, _I, _P]),
+    "kv_batch_set_data": (_I, [_P, _I, _P]),
+    "kv_batch_get_volume": (_I, [_P, _I, _P]),
+    "kv_batch_mlem": (_I, [_P, _P, _F, _I, ctypes.POINTER(_F)]),
+    "kv_batch_set_tiles": (_I, [_P, _I, _I, _I]),
+    "kv_batch_get_tiles": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "kv_batch_autotune": (_I, [_P, ctypes.c_char_p]),
 }
 
 
@@ -266,3 +275,87 @@ class Context:
         out = np.empty(g.vol_shape, dtype=np.float32)
         self._check(self.lib.kv_back_project_host(self.handle, _host_ptr(proj), _host_ptr(out), mode))
         return out
+
+
+def _geometry_struct(g: Geometry, keep: list) -> _KvGeometry:
+    det = np.ascontiguousarray(g.det, dtype=np.float32)
+    src = np.ascontiguousarray(g.src, dtype=np.float32)
+    keep += [det, src]
+    return _KvGeometry(g.nx, g.ny, g.nz, g.sx, g.sy, g.sz, g.nu, g.nv, g.nviews, g.pu, g.pv,
+                       det.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                       src.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+
+class Batch:
+    """A kv_batch: many mode-R MLEM reconstructions of uneven sizes advanced
+    together by persistent forward/backprojection launches, optionally
+    replayed from a captured hipGraph.  Host arrays in and out; every call is
+    synchronous."""
+
+    def __init__(self, geometries, device: int = 0):
+        self.lib = load_library()
+        self.geometries = list(geometries)
+        keep: list = []
+        arr = (_KvGeometry * len(self.geometries))(*[_geometry_struct(g, keep) for g in self.geometries])
+        h = _P()
+        self._check(self.lib.kv_batch_create(device, len(self.geometries), ctypes.cast(arr, _P), ctypes.byref(h)))
+        self.handle = h
+
+    def _check(self, status: int):
+        if status != KV_OK:
+            raise KvError(status, self.lib.kv_last_error().decode())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.kv_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.geometries)
+
+    def set_volume(self, i: int, vol: np.ndarray):
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        if vol.shape != tuple(self.geometries[i].vol_shape):
+            raise ValueError(f"volume {i}: shape {vol.shape} != {self.geometries[i].vol_shape}")
+        self._check(self.lib.kv_batch_set_volume(self.handle, i, _host_ptr(vol)))
+
+    def set_data(self, i: int, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        if data.shape != tuple(self.geometries[i].proj_shape):
+            raise ValueError(f"data {i}: shape {data.shape} != {self.geometries[i].proj_shape}")
+        self._check(self.lib.kv_batch_set_data(self.handle, i, _host_ptr(data)))
+
+    def get_volume(self, i: int) -> np.ndarray:
+        out = np.empty(self.geometries[i].vol_shape, dtype=np.float32)
+        self._check(self.lib.kv_batch_get_volume(self.handle, i, _host_ptr(out)))
+        return out
+
+    def mlem(self, iters, relax: float = 1.0, use_graph: bool = True) -> float:
+        """Run iters[i] MLEM iterations on reconstruction i (an int applies to
+        all); returns the device time in ms."""
+        if isinstance(iters, int):
+            iters = [iters] * len(self)
+        it = np.ascontiguousarray(iters, dtype=np.int32)
+        if it.shape != (len(self),):
+            raise ValueError("one iteration count per reconstruction")
+        ms = _F()
+        self._check(self.lib.kv_batch_mlem(self.handle, _host_ptr(it), relax, int(bool(use_graph)), ctypes.byref(ms)))
+        return ms.value
+
+    def set_tiles(self, i: int, fp_layout: int, bp_slices: int):
+        self._check(self.lib.kv_batch_set_tiles(self.handle, i, fp_layout, bp_slices))
+
+    def get_tiles(self, i: int):
+        a, b = _I(), _I()
+        self._check(self.lib.kv_batch_get_tiles(self.handle, i, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, cache_path: Optional[str] = None):
+        self._check(self.lib.kv_batch_autotune(self.handle, cache_path.encode() if cache_path else None))
+        return [self.get_tiles(i) for i in range(len(self))]

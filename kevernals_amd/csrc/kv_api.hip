@@ -150,6 +150,8 @@ bool almost_equal_relative(float a, float b)
 }
 }  // namespace
 
+kv_status kv_internal_ensure_tables(kv_context *ctx, int mode) { return ensure_tables(ctx, mode); }
+
 void kv_set_error(const char *fmt, ...)
 {
     char buf[512];

@@ -177,22 +177,34 @@ __device__ __forceinline__ bool shadow(float c, float s, float o, float t0, floa
     return true;
 }
 
+// LDS of one brick: table bands and the per-voxel running sums.
+template <int ZT>
+struct BpShared {
+    float xb[(TX + 1) * XBS];
+    int xbase[TX + 1];
+    float yb[(TY + 1) * YBR];
+    int ybase_s;
+    float accs[ZT][2][256];  // (value, weight) per slice of the brick
+};
+
+// Backprojection of brick (bxi, byi, bzi) of depth zt <= ZT slices.  Called
+// by all 256 threads of a workgroup (it synchronises).
 template <int MODE, int EP, int ZT, bool UNIT>
-__global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
-                                                       float *__restrict__ vol, const float *__restrict__ sens)
+__device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, const float *__restrict__ proj,
+                                         float *__restrict__ vol, const float *__restrict__ sens, int bxi, int byi,
+                                         int bzi, int zt, BpShared<ZT> &sm)
 {
-    __shared__ float xb[(TX + 1) * XBS];
-    __shared__ int xbase[TX + 1];
-    __shared__ float yb[(TY + 1) * YBR];
-    __shared__ int ybase_s;
-    // per-voxel running sums (value, weight), one pair per slice of the brick
-    __shared__ float accs[ZT][2][256];
+    float *xb = sm.xb;
+    int *xbase = sm.xbase;
+    float *yb = sm.yb;
+    int &ybase_s = sm.ybase_s;
+    auto &accs = sm.accs;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * ZT;
+    const int x0 = bxi * TX, y0 = byi * TY, z0 = bzi * zt;
     const int x = x0 + lane, y = y0 + wv;
-    const int nzt = min(ZT, g.nz - z0);
+    const int nzt = min(zt, g.nz - z0);
     const bool active = x < g.nx && y < g.ny;
     const float hx = (float)g.nx / 2.f, hy = (float)g.ny / 2.f, hz = (float)g.nz / 2.f;
     const float cvx = (float)x - hx, cvy = (float)y - hy;  // Projector.cu:437-440
@@ -437,6 +449,52 @@ __global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, con
 
 constexpr int kZT = 8;
 
+// One brick per workgroup.
+template <int MODE, int EP, int ZT, bool UNIT>
+__global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
+                                                       float *__restrict__ vol, const float *__restrict__ sens)
+{
+    __shared__ BpShared<ZT> sm;
+    bp_brick<MODE, EP, ZT, UNIT>(g, t, proj, vol, sens, blockIdx.x, blockIdx.y, blockIdx.z, ZT, sm);
+}
+
+// Persistent backprojection + MLEM update over a batch of reconstructions:
+// workgroups claim bricks from a global counter until the list is exhausted.
+__global__ __launch_bounds__(256) void bp_r_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+                                                         long long total, unsigned long long *counter, int iteration)
+{
+    __shared__ BpShared<kZT> sm;
+    __shared__ long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0)
+            item_s = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long item = item_s;
+        __syncthreads();
+        if (item >= total)
+            break;
+        int pi = 0;
+        while (pi + 1 < nprob && probs[pi + 1].bp_first <= item)
+            ++pi;
+        const KvProblem &P = probs[pi];
+        if (iteration >= P.iters)
+            continue;
+        const long long local = item - P.bp_first;
+        const int per_z = P.bp_bx * P.bp_by;
+        const int bz = (int)(local / per_z);
+        const int rem = (int)(local - (long long)bz * per_z);
+        const int by = rem / P.bp_bx, bx = rem - by * P.bp_bx;
+        bp_brick<KV_MODE_R, KV_EP_MLEM, kZT, false>(P.g, P.t, P.ratio, P.vol, nullptr, bx, by, bz, P.bp_zt, sm);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0ull;
+}
+
 template <int MODE, int EP, bool UNIT>
 hipError_t launch_tiled(const KvGeom &g, const KvTables &t, const float *proj, float *vol, const float *sens,
                         hipStream_t s)
@@ -447,6 +505,27 @@ hipError_t launch_tiled(const KvGeom &g, const KvTables &t, const float *proj, f
 }
 
 }  // namespace
+
+hipError_t kv_launch_bp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
+                                int iteration, int grid, hipStream_t s)
+{
+    bp_r_batch_kernel<<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s)
+{
+    zero_u64_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+int kv_bp_batch_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bp_r_batch_kernel, 256, 0) != hipSuccess)
+        return 2;
+    return n > 0 ? n : 1;
+}
 
 hipError_t kv_launch_tables_r(const KvGeom &g, const KvTables &t, hipStream_t s)
 {

@@ -189,15 +189,11 @@ __device__ __forceinline__ bool almost_equal_zero(float a)
     return diff <= __FLT_EPSILON__ * m;
 }
 
+// One ray of the mode-R forward pass with its epilogue.
 template <int EP>
-__global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
-                                                   const float *__restrict__ data, float relax)
+__device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restrict__ vol, float *__restrict__ out,
+                                         const float *__restrict__ data, float relax, int p, int u, int v)
 {
-    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int p = blockIdx.z;
-    if (u >= g.nu || v >= g.nv)
-        return;
     RayF r;
     ray_setup(g, p, u, v, r);
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
@@ -222,6 +218,64 @@ __global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__rest
     } else {
         // Reconstructors.h:90-93
         out[o] = relax * (data[o] - fp);
+    }
+}
+
+// Pixel of lane `tid` in forward tile (tx, ty) for tile layout L (1, 2 or 4):
+// each wavefront covers 64/L consecutive detector columns of L rows, the four
+// wavefronts of a workgroup are stacked along v; a tile is (64/L) x 4L pixels.
+__device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int tid, int &u, int &v)
+{
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wu = 64 / layout;
+    u = tx * wu + (lane % wu);
+    v = ty * (4 * layout) + wv * layout + lane / wu;
+}
+
+template <int EP>
+__global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
+                                                   const float *__restrict__ data, float relax)
+{
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (u >= g.nu || v >= g.nv)
+        return;
+    fp_r_ray<EP>(g, vol, out, data, relax, p, u, v);
+}
+
+// Persistent forward pass over a batch of reconstructions: workgroups claim
+// tiles from a global counter until the batch's tile list is exhausted.
+// Problems whose iteration count is reached skip their tiles.
+template <int EP>
+__global__ __launch_bounds__(256) void fp_r_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+                                                         long long total, unsigned long long *counter, int iteration,
+                                                         float relax)
+{
+    __shared__ long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0)
+            item_s = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long item = item_s;
+        __syncthreads();
+        if (item >= total)
+            break;
+        int pi = 0;
+        while (pi + 1 < nprob && probs[pi + 1].fp_first <= item)
+            ++pi;
+        const KvProblem &P = probs[pi];
+        if (iteration >= P.iters)
+            continue;
+        const long long local = item - P.fp_first;
+        const int per_view = P.fp_tx * P.fp_ty;
+        const int p = (int)(local / per_view);
+        const int rem = (int)(local - (long long)p * per_view);
+        const int ty = rem / P.fp_tx, tx = rem - ty * P.fp_tx;
+        int u, v;
+        fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
+        if (u < P.g.nu && v < P.g.nv)
+            fp_r_ray<EP>(P.g, P.vol, P.ratio, P.data, relax, p, u, v);
     }
 }
 
@@ -440,4 +494,19 @@ hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *
 {
     trace_kernel<<<(nrays + 63) / 64, 64, 0, s>>>(g, mode, nrays, rays, cap, idx, w, counts);
     return hipGetLastError();
+}
+
+hipError_t kv_launch_fp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
+                                int iteration, float relax, int grid, hipStream_t s)
+{
+    fp_r_batch_kernel<KV_EP_MLEM><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration, relax);
+    return hipGetLastError();
+}
+
+int kv_fp_batch_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fp_r_batch_kernel<KV_EP_MLEM>, 256, 0) != hipSuccess)
+        return 4;
+    return n > 0 ? n : 1;
 }

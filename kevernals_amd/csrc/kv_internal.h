@@ -39,6 +39,25 @@ struct KvTables {
 // fused SART residual/update, or the count of accepted (ray, voxel) pairs.
 enum KvEpilogue { KV_EP_PLAIN = 0, KV_EP_MLEM = 1, KV_EP_SART = 2, KV_EP_COUNT = 3 };
 
+// One reconstruction of a batch, as the persistent kernels read it (device
+// memory, wave-uniform loads).  Work items are numbered consecutively over
+// the batch: forward tiles [fp_first, fp_first + views*fp_tx*fp_ty) and
+// backprojection bricks [bp_first, bp_first + bp_bx*bp_by*bp_bz).
+struct KvProblem {
+    KvGeom g;
+    KvTables t;
+    float *vol;
+    const float *data;
+    float *ratio;
+    int iters;                 // MLEM iterations for this reconstruction
+    int fp_layout;             // forward tile layout: 1, 2 or 4 rows per wavefront
+    int fp_tx, fp_ty;          // forward tiles per view
+    long long fp_first;
+    int bp_zt;                 // backprojection brick depth: 4 or 8 slices
+    int bp_bx, bp_by, bp_bz;   // bricks per axis
+    long long bp_first;
+};
+
 struct kv_context {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -60,6 +79,9 @@ struct kv_context {
 
 // Thread-local error message.
 void kv_set_error(const char *fmt, ...);
+
+// Builds the crossing tables of `mode` for the context's geometry (once).
+kv_status kv_internal_ensure_tables(kv_context *ctx, int mode);
 
 #define KV_TRY_HIP(expr)                                                                        \
     do {                                                                                        \
@@ -87,6 +109,15 @@ hipError_t kv_launch_bp_r(const KvGeom &g, const KvTables &t, const float *proj,
                           hipStream_t s);
 hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
                           const float *sens, bool unit_input, hipStream_t s);
+// Batched persistent passes (MLEM epilogues, mode R).  `grid` workgroups of
+// 256 threads claim work items from *counter (zero before the launch).
+hipError_t kv_launch_fp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
+                                int iteration, float relax, int grid, hipStream_t s);
+hipError_t kv_launch_bp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
+                                int iteration, int grid, hipStream_t s);
+hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+int kv_fp_batch_blocks_per_cu();
+int kv_bp_batch_blocks_per_cu();
 hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes,
                          float *out, hipStream_t s);
 hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s);
