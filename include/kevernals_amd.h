@@ -178,6 +178,35 @@ kv_status kv_batch_get_tiles(kv_batch *b, int i, int *fp_layout, int *bp_slices)
  * written back.  Volumes are restored afterwards. */
 kv_status kv_batch_autotune(kv_batch *b, const char *cache_path);
 
+/* ---- learned step: 3D residual CNN (no reference counterpart) ------------
+ * y = x + f(s*x)/s, f = four 3x3x3 convolutions 1->16->16->16->1 with ReLU
+ * after the first three; bfloat16 weights and activations, FP32
+ * accumulation on the matrix cores.  Parameters are one FP32 blob of
+ * KV_CNN_NPARAMS values: W1[16][1][27] b1[16] W2[16][16][27] b2[16]
+ * W3[16][16][27] b3[16] W4[1][16][27] b4[1], W[o][i][t] with tap
+ * t = 9(dz+1) + 3(dy+1) + (dx+1).  Volumes float[nz][ny][nx].  The caller
+ * provides a 256-byte aligned device workspace of kv_cnn_workspace_size
+ * bytes.  Asynchronous on the context's stream unless stated. */
+#define KV_CNN_NPARAMS 14737
+kv_status kv_cnn_workspace_size(int nx, int ny, int nz, int training, size_t *bytes);
+/* Byte offsets in a training workspace of the stored intermediates
+ * a1, a2, a3 (bf16 [V][16]), dz4 (float [V], dL/dz4), dz3, dz2, dz1
+ * (bf16 [V][16], gradients after the ReLU masks): offsets[0..6]. */
+kv_status kv_cnn_workspace_offsets(int nx, int ny, int nz, size_t *offsets);
+/* Inference: d_y = d_x + f(scale*d_x)/scale (d_y must not alias d_x). */
+kv_status kv_cnn_forward(kv_context *ctx, const float *d_params, float scale, const float *d_x, float *d_y, int nx,
+                         int ny, int nz, void *d_work, size_t work_bytes);
+/* Loss L = mean((scale*(y - target))^2) and its gradient dL/dparams into
+ * d_grad (KV_CNN_NPARAMS floats).  Deterministic.  Synchronous when loss is
+ * non-NULL (the loss is copied back). */
+kv_status kv_cnn_gradient(kv_context *ctx, const float *d_params, float scale, const float *d_x,
+                          const float *d_target, int nx, int ny, int nz, void *d_work, size_t work_bytes,
+                          float *d_grad, double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected, step >= 1)
+ * on the parameter blob; d_m and d_v are the moment buffers (start at 0). */
+kv_status kv_cnn_adam(kv_context *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, int step,
+                      float lr);
+
 /* ---- host-buffer variants (synchronous; used by the C++ Projector) ------- */
 kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode);
 kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode);

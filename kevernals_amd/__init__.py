@@ -90,6 +90,12 @@ _SIGS = {
     "kv_batch_set_tiles": (_I, [_P, _I, _I, _I]),
     "kv_batch_get_tiles": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "kv_batch_autotune": (_I, [_P, ctypes.c_char_p]),
+    "kv_cnn_workspace_size": (_I, [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "kv_cnn_workspace_offsets": (_I, [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "kv_cnn_forward": (_I, [_P, _P, _F, _P, _P, _I, _I, _I, _P, ctypes.c_size_t]),
+    "kv_cnn_gradient": (_I, [_P, _P, _F, _P, _P, _I, _I, _I, _P, ctypes.c_size_t, _P,
+                             ctypes.POINTER(ctypes.c_double)]),
+    "kv_cnn_adam": (_I, [_P, _P, _P, _P, _P, _I, _F]),
 }
 
 
@@ -131,6 +137,35 @@ def _ptr(x) -> int:
             raise ValueError("tensor must be contiguous")
         return x.data_ptr()
     raise TypeError(f"cannot take a device address of {type(x)}")
+
+
+def _nbytes(x) -> int:
+    return x.numel() * x.element_size()
+
+
+CNN_NPARAMS = 14737
+
+
+def cnn_workspace_size(shape, training: bool) -> int:
+    """Bytes of CNN workspace for a volume of shape (nz, ny, nx)."""
+    lib = load_library()
+    nz, ny, nx = shape
+    n = ctypes.c_size_t()
+    st = lib.kv_cnn_workspace_size(nx, ny, nz, int(training), ctypes.byref(n))
+    if st != KV_OK:
+        raise KvError(st, lib.kv_last_error().decode())
+    return n.value
+
+
+def cnn_workspace_offsets(shape):
+    """Byte offsets of a1, a2, a3, dz4, dz3, dz2, dz1 in a training workspace."""
+    lib = load_library()
+    nz, ny, nx = shape
+    o = (ctypes.c_size_t * 7)()
+    st = lib.kv_cnn_workspace_offsets(nx, ny, nz, o)
+    if st != KV_OK:
+        raise KvError(st, lib.kv_last_error().decode())
+    return list(o)
 
 
 def _host_ptr(a: np.ndarray) -> int:
@@ -262,6 +297,24 @@ class Context:
         return [(idx[i, : min(counts[i], cap)], w[i, : min(counts[i], cap)], int(counts[i])) for i in range(n)]
 
     # -- host-buffer variants --------------------------------------------
+    # -- learned step (3D residual CNN) --------------------------------------
+    def cnn_forward(self, params, scale: float, x, y, work):
+        """y = x + f(scale*x)/scale for a volume x of shape (nz, ny, nx)."""
+        nz, ny, nx = x.shape
+        self._dev(self.lib.kv_cnn_forward, _ptr(params), scale, _ptr(x), _ptr(y), nx, ny, nz, _ptr(work),
+                  _nbytes(work))
+
+    def cnn_gradient(self, params, scale: float, x, target, work, grad) -> float:
+        """Gradient of mean((scale*(y - target))^2) into grad; returns the loss."""
+        nz, ny, nx = x.shape
+        loss = ctypes.c_double()
+        self._dev(self.lib.kv_cnn_gradient, _ptr(params), scale, _ptr(x), _ptr(target), nx, ny, nz, _ptr(work),
+                  _nbytes(work), _ptr(grad), ctypes.byref(loss))
+        return loss.value
+
+    def cnn_adam(self, params, grad, m, v, step: int, lr: float = 1e-3):
+        self._dev(self.lib.kv_cnn_adam, _ptr(params), _ptr(grad), _ptr(m), _ptr(v), step, lr)
+
     def forward_project_host(self, vol: np.ndarray, mode: int = MODE_R) -> np.ndarray:
         g = self.geometry
         vol = np.ascontiguousarray(vol, dtype=np.float32)

@@ -1,0 +1,183 @@
+"""Learned-accelerated MLEM: training pairs, CNN training and the
+likelihood-gated interleaving of CNN steps with MLEM iterations.
+
+Everything numerical runs on the GPU through the C ABI (kv_mlem_iteration
+in mode A, kv_log_likelihood, kv_cnn_*); this module only sequences calls.
+The reconstruction is standard MLEM on line-integral data,
+x <- x / A^T 1 * A^T(data / Ax), the mode-A form of recons::MLEM
+(KevLoq/Kevernals Reconstructors.h:182-242).  The CNN maps an early MLEM
+estimate towards the 100-iteration one; a CNN step is kept only if it does
+not lower the Poisson log-likelihood sum(data*log(Ax) - Ax), so plain MLEM
+stays both the fallback and the judge.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import CNN_NPARAMS, MODE_A, Context, cnn_workspace_size
+
+DEFAULT_SCALE = 20.0  # attenuation values (1/mm) are O(0.02-0.1): s*x is O(1)
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights (fan-in 27*cin), zero biases, last layer x0.1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for li, (co, ci) in enumerate([(16, 1), (16, 16), (16, 16), (1, 16)]):
+        w = rng.standard_normal((co, ci, 27)) * np.sqrt(2.0 / (27 * ci))
+        if li == 3:
+            w *= 0.1
+        out += [w.ravel(), np.zeros(co)]
+    p = np.concatenate(out).astype(np.float32)
+    assert p.size == CNN_NPARAMS
+    return p
+
+
+class Cnn:
+    """Parameters, Adam state and workspaces of the residual CNN on one
+    device; all calls are enqueued on the context's (torch's current)
+    stream."""
+
+    def __init__(self, ctx: Context, params: Optional[np.ndarray] = None, scale: float = DEFAULT_SCALE,
+                 seed: int = 0):
+        import torch
+
+        self.torch = torch
+        self.ctx = ctx
+        self.scale = float(scale)
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != CNN_NPARAMS:
+            raise ValueError(f"expected {CNN_NPARAMS} parameters, got {p.size}")
+        self.params = torch.from_numpy(p.copy()).cuda()
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.step = 0
+        self._work = {}
+
+    def _workspace(self, shape, training):
+        key = (tuple(shape), bool(training))
+        w = self._work.get(key)
+        if w is None:
+            w = self.torch.empty(cnn_workspace_size(shape, training), dtype=self.torch.uint8, device="cuda")
+            self._work[key] = w
+        return w
+
+    def forward(self, x, y):
+        """y = x + f(s*x)/s (device tensors of shape (nz, ny, nx))."""
+        self.ctx.cnn_forward(self.params, self.scale, x, y, self._workspace(x.shape, False))
+
+    def loss_and_grad(self, x, target) -> float:
+        return self.ctx.cnn_gradient(self.params, self.scale, x, target, self._workspace(x.shape, True), self.grad)
+
+    def train_step(self, x, target, lr: float = 1e-3) -> float:
+        loss = self.loss_and_grad(x, target)
+        self.step += 1
+        self.ctx.cnn_adam(self.params, self.grad, self.m, self.v, self.step, lr)
+        return loss
+
+    def params_host(self) -> np.ndarray:
+        return self.params.cpu().numpy()
+
+    def save(self, path: str, **meta):
+        np.savez(path, params=self.params_host(), scale=np.float32(self.scale), steps=np.int64(self.step),
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Cnn":
+        with np.load(path, allow_pickle=False) as f:
+            c = cls(ctx, f["params"], float(f["scale"]))
+            c.step = int(f["steps"])
+        return c
+
+
+def mlem_a(ctx: Context, x, data, iters: int):
+    for _ in range(iters):
+        ctx.mlem_iteration(x, data, 1.0, MODE_A)
+
+
+def transmission_data(ctx: Context, mu, i0: float, generator):
+    """Noisy line integrals -ln(max(I,1)/I0), I ~ Poisson(I0 exp(-A mu)),
+    sampled on the device."""
+    torch = __import__("torch")
+    g = ctx.geometry
+    ell = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(mu, ell, MODE_A)
+    counts = torch.poisson(i0 * torch.exp(-ell), generator=generator)
+    return -torch.log(torch.clamp(counts, min=1.0) / i0)
+
+
+def rmse(x, ref) -> float:
+    """||x - ref|| / ||ref|| over the volume (FP64 accumulation)."""
+    d = (x.double() - ref.double()).norm()
+    return float(d / ref.double().norm())
+
+
+@dataclass
+class GatedRun:
+    x: object
+    trace: List[tuple] = field(default_factory=list)  # (kind, iteration, seconds, rmse or None, loglik or None)
+    accepted: int = 0
+    rejected: int = 0
+
+
+def gated_mlem(ctx: Context, cnn: Optional[Cnn], data, iters: int, every: int = 2, first: int = 2,
+               reference=None, target_rmse: Optional[float] = None, x0=None) -> GatedRun:
+    """MLEM-A from a uniform (or given) start with a CNN candidate after
+    iteration `first` and every `every` iterations after it; a candidate
+    replaces the estimate only if its log-likelihood is not lower.  With a
+    reference volume the RMSE after every step is traced; with target_rmse
+    the run stops as soon as it is reached.  Wall times include the
+    likelihood gate and are taken with the device synchronised."""
+    torch = __import__("torch")
+    g = ctx.geometry
+    x = torch.ones(g.vol_shape, device="cuda") if x0 is None else x0.clone()
+    cand = torch.empty_like(x) if cnn is not None else None
+    run = GatedRun(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paused = 0.0
+
+    def note(kind, k, ll=None):
+        nonlocal paused
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0 - paused
+        r = None
+        if reference is not None:
+            tp = time.perf_counter()
+            r = rmse(x, reference)
+            paused += time.perf_counter() - tp
+        run.trace.append((kind, k, t, r, ll))
+        return r
+
+    for k in range(1, iters + 1):
+        ctx.mlem_iteration(x, data, 1.0, MODE_A)
+        r = note("mlem", k)
+        if target_rmse is not None and r is not None and r <= target_rmse:
+            break
+        if cnn is not None and k >= first and (k - first) % every == 0:
+            cnn.forward(x, cand)
+            ll_x = ctx.log_likelihood(data, x)
+            ll_c = ctx.log_likelihood(data, cand)
+            if ll_c >= ll_x:
+                x.copy_(cand)
+                run.accepted += 1
+                r = note("cnn+", k, ll_c)
+            else:
+                run.rejected += 1
+                r = note("cnn-", k, ll_x)
+            if target_rmse is not None and r is not None and r <= target_rmse:
+                break
+    run.x = x
+    return run
+
+
+def time_to_rmse(trace, target: float) -> Optional[float]:
+    for kind, k, t, r, ll in trace:
+        if r is not None and r <= target:
+            return t
+    return None

@@ -1,0 +1,127 @@
+"""Train the residual CNN of the learned MLEM step on pairs the method makes
+itself: 5-iteration MLEM (input) and 100-iteration MLEM (target) of seeded
+phantoms, 256x256x32 voxels at 0.25x0.25x2 mm, 512^2 detector at 0.3 mm,
+15 views on +-15 deg, transmission data at I0 = 1e4.  L2 loss, Adam.
+
+    python -m kevernals_amd.train_cnn --pairs 2000 --epochs 4 --out weights.npz
+
+Phantoms are rasterised by CPU worker processes started before the GPU is
+touched; projection, Poisson sampling, MLEM and training run on the GPU
+and all pairs stay resident in device memory (16 MiB each).
+"""
+from __future__ import annotations
+
+import argparse
+import multiprocessing as mp
+import sys
+import time
+
+import numpy as np
+
+TRAIN_SEED0 = 4000
+VAL_SEED0 = 6000
+
+
+def _phantom(args):
+    from kevernals_amd import geometry as G
+    from kevernals_amd.phantom import phantom
+
+    return phantom(G.learned_train(), args)
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def make_pairs(ctx, seeds, pool, i0, k_in, k_ref, gen):
+    import torch
+
+    from kevernals_amd.learned import transmission_data
+
+    g = ctx.geometry
+    xs, ts = [], []
+    t0 = time.time()
+    for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
+        mu = torch.from_numpy(mu_h).cuda()
+        data = transmission_data(ctx, mu, i0, gen)
+        x = torch.ones(g.vol_shape, device="cuda")
+        for _ in range(k_in):
+            ctx.mlem_iteration(x, data, 1.0, 1)
+        xs.append(x.clone())
+        for _ in range(k_ref - k_in):
+            ctx.mlem_iteration(x, data, 1.0, 1)
+        ts.append(x)
+        if (n + 1) % 50 == 0 or n + 1 == len(seeds):
+            torch.cuda.synchronize()
+            log(f"[train] pairs {n + 1}/{len(seeds)} in {time.time() - t0:.1f} s")
+    return xs, ts
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--val", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--i0", type=float, default=1e4)
+    ap.add_argument("--k-in", type=int, default=5)
+    ap.add_argument("--k-ref", type=int, default=100)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="cnn_weights.npz")
+    args = ap.parse_args(argv)
+
+    # CPU workers first: no process is started after the GPU is initialised
+    pool = mp.get_context("fork").Pool(args.workers)
+
+    import torch
+
+    import kevernals_amd as K
+    from kevernals_amd import geometry as G
+    from kevernals_amd.learned import Cnn
+
+    g = G.learned_train()
+    ctx = K.Context(0, g)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed + 12345)
+    t0 = time.time()
+    xs, ts = make_pairs(ctx, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs)), pool, args.i0, args.k_in,
+                        args.k_ref, gen)
+    vx, vt = make_pairs(ctx, list(range(VAL_SEED0, VAL_SEED0 + args.val)), pool, args.i0, args.k_in, args.k_ref,
+                        gen)
+    pool.close()
+    pool.join()
+    t_data = time.time() - t0
+    log(f"[train] {args.pairs} + {args.val} pairs ready in {t_data:.1f} s")
+
+    cnn = Cnn(ctx, seed=args.seed)
+
+    def val_loss():
+        return float(np.mean([cnn.loss_and_grad(a, b) for a, b in zip(vx, vt)]))
+
+    def identity_loss():
+        s = cnn.scale
+        return float(np.mean([float(((a - b).double() * s).pow(2).mean()) for a, b in zip(vx, vt)]))
+
+    base = identity_loss()
+    log(f"[train] validation loss of the 5-iteration input itself: {base:.5e}; untrained CNN {val_loss():.5e}")
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    history = []
+    t1 = time.time()
+    for ep in range(args.epochs):
+        order = rng.permutation(len(xs))
+        tl = []
+        for i in order:
+            tl.append(cnn.train_step(xs[i], ts[i], args.lr))
+        vl = val_loss()
+        history.append((ep + 1, float(np.mean(tl)), vl))
+        log(f"[train] epoch {ep + 1}: train {np.mean(tl):.5e} val {vl:.5e} ({time.time() - t1:.1f} s)")
+    steps_per_s = cnn.step / (time.time() - t1)
+    cnn.save(args.out, history=np.array(history), identity_val=base, pairs=args.pairs, k_in=args.k_in,
+             k_ref=args.k_ref, i0=args.i0, data_seconds=t_data, steps_per_s=steps_per_s)
+    log(f"[train] saved {args.out}: {cnn.step} steps, {steps_per_s:.1f} steps/s")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
