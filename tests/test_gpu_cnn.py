@@ -139,9 +139,12 @@ def test_forward_matches_training_pass_and_oracle(K, oracle_mod):
     err = np.abs(got - ref_bf)
     print(f"[cnn] forward vs bf16 oracle: max err {err.max():.3e} (max correction {corr.max():.3e}); "
           f"bf16 vs fp64 network: max {np.abs(ref_bf - ref64).max():.3e}", flush=True)
-    # rounding flips of single bf16 activations propagate: bound relative to the correction
-    assert err.max() <= 2e-2 * corr.max()
-    assert np.mean(err <= 1e-6 * np.abs(ref_bf) + 1e-4 * corr.max()) >= 0.99
+    # a rounding flip of one bf16 activation (a tie decided differently by
+    # FP32 and FP64 sums) spreads through the later layers' 3x3x3 windows to
+    # a few percent of the outputs: bound the worst case relative to the
+    # network's correction and require the typical voxel to agree closely
+    assert err.max() <= 5e-3 * corr.max()
+    assert np.median(err) <= 1e-5 * corr.max()
     # the inference pass equals the training pass's output
     loss, _, work_t = run_gradient(K, x, t, params)
     V = x.size
