@@ -101,14 +101,16 @@ def mlem_a(ctx: Context, x, data, iters: int):
 
 
 def transmission_data(ctx: Context, mu, i0: float, generator):
-    """Noisy line integrals -ln(max(I,1)/I0), I ~ Poisson(I0 exp(-A mu)),
-    sampled on the device."""
+    """Noisy line integrals -ln(clamp(I, 1, I0)/I0), I ~ Poisson(I0 exp(-A mu)),
+    sampled on the device.  Counts above I0 are clamped: the line integrals
+    stay non-negative, as the multiplicative MLEM update requires (negative
+    data drive estimates negative and the iteration unstable)."""
     torch = __import__("torch")
     g = ctx.geometry
     ell = torch.empty(g.proj_shape, device="cuda")
     ctx.forward_project(mu, ell, MODE_A)
     counts = torch.poisson(i0 * torch.exp(-ell), generator=generator)
-    return -torch.log(torch.clamp(counts, min=1.0) / i0)
+    return -torch.log(torch.clamp(counts, min=1.0, max=i0) / i0)
 
 
 def rmse(x, ref) -> float:
@@ -126,13 +128,16 @@ class GatedRun:
 
 
 def gated_mlem(ctx: Context, cnn: Optional[Cnn], data, iters: int, every: int = 2, first: int = 2,
-               reference=None, target_rmse: Optional[float] = None, x0=None) -> GatedRun:
+               reference=None, target_rmse: Optional[float] = None, x0=None,
+               patience: Optional[int] = None) -> GatedRun:
     """MLEM-A from a uniform (or given) start with a CNN candidate after
     iteration `first` and every `every` iterations after it; a candidate
-    replaces the estimate only if its log-likelihood is not lower.  With a
-    reference volume the RMSE after every step is traced; with target_rmse
-    the run stops as soon as it is reached.  Wall times include the
-    likelihood gate and are taken with the device synchronised."""
+    replaces the estimate only if its log-likelihood is not lower.  After
+    `patience` consecutive rejections no further candidates are made (None:
+    never stop).  With a reference volume the RMSE after every step is
+    traced; with target_rmse the run stops as soon as it is reached.  Wall
+    times include the likelihood gate and are taken with the device
+    synchronised (the RMSE evaluation itself is excluded)."""
     torch = __import__("torch")
     g = ctx.geometry
     x = torch.ones(g.vol_shape, device="cuda") if x0 is None else x0.clone()
@@ -154,21 +159,25 @@ def gated_mlem(ctx: Context, cnn: Optional[Cnn], data, iters: int, every: int = 
         run.trace.append((kind, k, t, r, ll))
         return r
 
+    misses = 0
     for k in range(1, iters + 1):
         ctx.mlem_iteration(x, data, 1.0, MODE_A)
         r = note("mlem", k)
         if target_rmse is not None and r is not None and r <= target_rmse:
             break
-        if cnn is not None and k >= first and (k - first) % every == 0:
+        active = cnn is not None and (patience is None or misses < patience)
+        if active and k >= first and (k - first) % every == 0:
             cnn.forward(x, cand)
             ll_x = ctx.log_likelihood(data, x)
             ll_c = ctx.log_likelihood(data, cand)
             if ll_c >= ll_x:
                 x.copy_(cand)
                 run.accepted += 1
+                misses = 0
                 r = note("cnn+", k, ll_c)
             else:
                 run.rejected += 1
+                misses += 1
                 r = note("cnn-", k, ll_x)
             if target_rmse is not None and r is not None and r <= target_rmse:
                 break

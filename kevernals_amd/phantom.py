@@ -93,8 +93,10 @@ def phantom(g: Geometry, seed: int, microcalcifications: int = 0) -> np.ndarray:
 
 
 def poisson_line_integrals(ell: np.ndarray, i0: float, seed: int) -> np.ndarray:
-    """Noisy line integrals -ln(max(I,1)/I0), I ~ Poisson(I0 exp(-ell))."""
+    """Noisy line integrals -ln(clamp(I, 1, I0)/I0), I ~ Poisson(I0 exp(-ell)).
+    Counts above I0 (short rays) are clamped so that the line integrals stay
+    non-negative, as the multiplicative MLEM update requires."""
     rng = _rng(seed)
     lam = i0 * np.exp(-ell.astype(np.float64))
     counts = rng.poisson(lam)
-    return (-np.log(np.maximum(counts, 1) / i0)).astype(np.float32)
+    return (-np.log(np.clip(counts, 1, i0) / i0)).astype(np.float32)
