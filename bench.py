@@ -181,6 +181,60 @@ def bench_batched(args, torch, K, G):
     print(json.dumps(out), flush=True)
 
 
+CNN_FLOP_PER_VOXEL = 2.0 * 27 * (16 + 256 + 256 + 16)  # one forward application
+
+
+def bench_cnn(args, torch, K, G):
+    """The CNN alone: inference on a 1024x1024x64 volume and training steps
+    (forward + backward + Adam) on 256x256x32, against the dense BF16 MFMA
+    peak.  Algorithmic FLOPs: 2*27*(1*16 + 16*16 + 16*16 + 16*1) per voxel
+    forward; a training step counts 3x (forward, input and weight
+    gradients)."""
+    from kevernals_amd.learned import Cnn
+
+    ctx = K.Context(0)
+    cnn = Cnn(ctx, seed=1)
+    res = {}
+    for name, shape, train in (("inference", (64, 1024, 1024), False), ("training", (32, 256, 256), True)):
+        x = 0.02 + 0.05 * torch.rand(shape, device="cuda")
+        y = torch.empty_like(x)
+        tgt = x + 0.01 * torch.randn(shape, device="cuda")
+
+        def step():
+            if train:
+                cnn.train_step(x, tgt, 1e-4)
+            else:
+                cnn.forward(x, y)
+
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        nvox = shape[0] * shape[1] * shape[2]
+        flops = CNN_FLOP_PER_VOXEL * nvox * (3 if train else 1)
+        tfs = flops / (ms * 1e-3) / 1e12
+        res[name] = {"shape": list(shape), "ms": round(ms, 3), "tflops": round(tfs, 1),
+                     "frac_of_bf16_peak": round(tfs / 2500.0, 4)}
+        log(f"[bench] cnn {name} {shape}: {ms:.3f} ms, {tfs:.1f} TFLOP/s")
+        del x, y, tgt
+    out = {
+        "metric": METRIC, "value": res["inference"]["tflops"], "unit": "TFLOP/s (CNN inference, dense bf16)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["inference"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (f32 accumulate)",
+        "data": "synthetic (uniform random volumes)",
+        "config": {"workload": "3D residual CNN 1-16-16-16-1, 3x3x3", "parallelism": "single GPU"},
+        "cnn": res,
+        "roofline": {"bound": "mfma", "achieved": res["inference"]["tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": res["inference"]["frac_of_bf16_peak"], "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned(args, torch, K, G):
     """Time to RMSE <= 1% against the 100-iteration MLEM result, plain MLEM
     vs MLEM with the likelihood-gated CNN (candidate after iteration 5 and
@@ -266,7 +320,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["full", "small", "batched", "learned"], default="full")
+    ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "cnn"], default="full")
     ap.add_argument("--tile-cache", default=None, help="autotuner cache file (batched workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an elementwise copy of 256 MiB (known bytes) to scale HBM counters")
@@ -285,6 +339,8 @@ def main():
         return bench_batched(args, torch, K, G)
     if args.workload == "learned":
         return bench_learned(args, torch, K, G)
+    if args.workload == "cnn":
+        return bench_cnn(args, torch, K, G)
     g = G.full_plain() if args.workload == "full" else G.small_plain()
     ctx = K.Context(0, g)
     ctx.use_torch_stream()

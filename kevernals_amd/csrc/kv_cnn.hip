@@ -109,11 +109,50 @@ __device__ __forceinline__ void store4(const f32x4 &acc, int g, size_t v, const 
     *reinterpret_cast<uint2 *>(out16 + v * 16 + 4 * g) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
 }
 
+// Weight operands of every convolution, packed once per call as the MFMA A
+// operands (bf16): variant v, MFMA m, lane l at wpk[(v * 14 + m) * 64 + l].
+// conv16 variants use m = 0..13 (k-group g = l >> 4 holds tap 2m + (g >> 1),
+// channels 8(g & 1) .. +7); conv1 variants use m = 0 (k-group g holds taps
+// 8g .. 8g+7).
+enum { WV_L2 = 0, WV_L3 = 1, WV_L4 = 2, WV_L3T = 3, WV_L2T = 4, WV_L1 = 5, WV_L4T = 6, WV_COUNT = 7 };
+
+__global__ void pack_weights_kernel(const float *__restrict__ p, u32x4 *__restrict__ wpk, int ow1, int ow2, int ow3,
+                                    int ow4)
+{
+    const int v = blockIdx.x, lane = threadIdx.x, col = lane & 15, g = lane >> 4;
+    const float *w;
+    int cout, cin, tr;
+    switch (v) {
+    case WV_L2: w = p + ow2; cout = 16; cin = 16; tr = 0; break;
+    case WV_L3: w = p + ow3; cout = 16; cin = 16; tr = 0; break;
+    case WV_L4: w = p + ow4; cout = 1; cin = 16; tr = 0; break;
+    case WV_L3T: w = p + ow3; cout = 16; cin = 16; tr = 1; break;
+    case WV_L2T: w = p + ow2; cout = 16; cin = 16; tr = 1; break;
+    case WV_L1: w = p + ow1; cout = 16; cin = 1; tr = 0; break;
+    default: w = p + ow4; cout = 1; cin = 16; tr = 1; break;
+    }
+    for (int m = 0; m < 14; ++m) {
+        unsigned int e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float val;
+            if (v >= WV_L1)
+                val = m == 0 ? w_elem(w, cout, cin, tr, col, 0, 8 * g + j) : 0.f;
+            else
+                val = w_elem(w, cout, cin, tr, col, 8 * (g & 1) + j, 2 * m + (g >> 1));
+            e[j] = f2bf(val);
+        }
+        wpk[(v * 14 + m) * 64 + lane] = pack8(e);
+    }
+}
+
+constexpr int ZPB = 4;  // z-chunks of BZ slices per workgroup (weights loaded once per workgroup)
+
 // 16 -> 16 (or 16 -> 1) channel convolution of a bf16 [V][16] input.
 template <int EP>
 __global__ __launch_bounds__(256) void conv16_kernel(int nx, int ny, int nz, const u32x4 *__restrict__ in,
-                                                     const float *__restrict__ w, const float *__restrict__ bias,
-                                                     int cout, int cin, int tr, unsigned short *__restrict__ out16,
+                                                     const u32x4 *__restrict__ wpk, const float *__restrict__ bias,
+                                                     unsigned short *__restrict__ out16,
                                                      const unsigned short *__restrict__ mask,
                                                      const float *__restrict__ x, const float *__restrict__ target,
                                                      float *__restrict__ outf, float s, float gscale,
@@ -123,57 +162,61 @@ __global__ __launch_bounds__(256) void conv16_kernel(int nx, int ny, int nz, con
     __shared__ double red[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int col = lane & 15, g = lane >> 4;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY, z0 = blockIdx.z * BZ;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
 
-    for (int c = tid; c < 2 * HVOX; c += 256) {
-        const int q = c >> 1, h = c & 1;
-        const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
-        const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
-        u32x4 v = {0u, 0u, 0u, 0u};
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-            v = in[vidx(gx, gy, gz, nx, ny) * 2 + h];
-        lds[h * HVOX + q] = v;
-    }
-
-    // A operands (weights) and B offsets of the 14 MFMAs of a row:
-    // k-group g of lane holds tap 2m + (g >> 1), channels 8(g & 1) .. +7.
+    // A operands (weights) and B offsets of the 14 MFMAs of a row
     u32x4 A[14];
     int off[14];
 #pragma unroll
     for (int m = 0; m < 14; ++m) {
+        A[m] = wpk[m * 64 + lane];
         const int t = 2 * m + (g >> 1);
-        unsigned int e[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            e[j] = f2bf(w_elem(w, cout, cin, tr, col, 8 * (g & 1) + j, t));
-        A[m] = pack8(e);
         const int tt = t < 27 ? t : 13;  // the padding tap has zero weights
         off[m] = (g & 1) * HVOX + (tap_dz(tt) + 1) * HYX + (tap_dy(tt) + 1) * HX + (tap_dx(tt) + 1) + col;
     }
-    __syncthreads();
 
     double lsum = 0.0;
-    for (int r = wv; r < BY * BZ; r += 4) {
-        const int ry = r % BY, rz = r / BY;
-        const int base = rz * HYX + ry * HX;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int zc_end = min((int)(blockIdx.z + 1) * ZPB, (nz + BZ - 1) / BZ);
+    for (int zc = blockIdx.z * ZPB; zc < zc_end; ++zc) {
+        const int z0 = zc * BZ;
+        __syncthreads();  // the previous chunk's operands are consumed
+        // one voxel (both 16-byte channel halves) per thread: contiguous
+        // 32-byte global reads, consecutive 16-byte LDS writes per plane
+        for (int q = tid; q < HVOX; q += 256) {
+            const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
+            const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
+            u32x4 v0 = {0u, 0u, 0u, 0u}, v1 = {0u, 0u, 0u, 0u};
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+                const size_t vi = vidx(gx, gy, gz, nx, ny) * 2;
+                v0 = in[vi];
+                v1 = in[vi + 1];
+            }
+            lds[q] = v0;
+            lds[HVOX + q] = v1;
+        }
+        __syncthreads();
+        for (int r = wv; r < BY * BZ; r += 4) {
+            const int ry = r % BY, rz = r / BY;
+            const int base = rz * HYX + ry * HX;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int m = 0; m < 14; ++m)
-            acc = mfma(A[m], lds[base + off[m]], acc);
-        const int gx = x0 + col, gy = y0 + ry, gz = z0 + rz;
-        if (gx >= nx || gy >= ny || gz >= nz)
-            continue;
-        const size_t v = vidx(gx, gy, gz, nx, ny);
-        if (EP == EP_RELU || EP == EP_MASK) {
-            store4<EP>(acc, g, v, bias, out16, mask);
-        } else if (g == 0) {
-            const float z4 = acc[0] + bias[0];
-            if (EP == EP_OUT) {
-                outf[v] = x[v] + z4 / s;
-            } else {
-                const float res = (s * x[v] + z4) - s * target[v];
-                outf[v] = res * gscale;
-                lsum += (double)res * (double)res;
+            for (int m = 0; m < 14; ++m)
+                acc = mfma(A[m], lds[base + off[m]], acc);
+            const int gx = x0 + col, gy = y0 + ry, gz = z0 + rz;
+            if (gx >= nx || gy >= ny || gz >= nz)
+                continue;
+            const size_t v = vidx(gx, gy, gz, nx, ny);
+            if (EP == EP_RELU || EP == EP_MASK) {
+                store4<EP>(acc, g, v, bias, out16, mask);
+            } else if (g == 0) {
+                const float z4 = acc[0] + bias[0];
+                if (EP == EP_OUT) {
+                    outf[v] = x[v] + z4 / s;
+                } else {
+                    const float res = (s * x[v] + z4) - s * target[v];
+                    outf[v] = res * gscale;
+                    lsum += (double)res * (double)res;
+                }
             }
         }
     }
@@ -196,45 +239,47 @@ __global__ __launch_bounds__(256) void conv16_kernel(int nx, int ny, int nz, con
 // last layer (input dz4, ReLU mask of a3).
 template <int EP>
 __global__ __launch_bounds__(256) void conv1_kernel(int nx, int ny, int nz, const float *__restrict__ in, float sc,
-                                                    const float *__restrict__ w, const float *__restrict__ bias,
-                                                    int cout, int cin, int tr, unsigned short *__restrict__ out16,
+                                                    const u32x4 *__restrict__ wpk, const float *__restrict__ bias,
+                                                    unsigned short *__restrict__ out16,
                                                     const unsigned short *__restrict__ mask)
 {
     __shared__ unsigned short lds[HVOX];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int col = lane & 15, g = lane >> 4;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY, z0 = blockIdx.z * BZ;
-    for (int q = tid; q < HVOX; q += 256) {
-        const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
-        const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
-        unsigned int v = 0u;
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-            v = f2bf(sc * in[vidx(gx, gy, gz, nx, ny)]);
-        lds[q] = (unsigned short)v;
-    }
-    // k-group g of lane holds taps 8g .. 8g+7 (27..31 are zero)
-    unsigned int e[8];
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    const u32x4 A = wpk[lane];
     int off[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int t = 8 * g + j;
-        e[j] = f2bf(w_elem(w, cout, cin, tr, col, 0, t));
         const int tt = t < 27 ? t : 13;
         off[j] = (tap_dz(tt) + 1) * HYX + (tap_dy(tt) + 1) * HX + (tap_dx(tt) + 1) + col;
     }
-    const u32x4 A = pack8(e);
-    __syncthreads();
-    for (int r = wv; r < BY * BZ; r += 4) {
-        const int ry = r % BY, rz = r / BY;
-        const int base = rz * HYX + ry * HX;
-        unsigned int b[8];
+    const int zc_end = min((int)(blockIdx.z + 1) * ZPB, (nz + BZ - 1) / BZ);
+    for (int zc = blockIdx.z * ZPB; zc < zc_end; ++zc) {
+        const int z0 = zc * BZ;
+        __syncthreads();
+        for (int q = tid; q < HVOX; q += 256) {
+            const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
+            const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
+            unsigned int v = 0u;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+                v = f2bf(sc * in[vidx(gx, gy, gz, nx, ny)]);
+            lds[q] = (unsigned short)v;
+        }
+        __syncthreads();
+        for (int r = wv; r < BY * BZ; r += 4) {
+            const int ry = r % BY, rz = r / BY;
+            const int base = rz * HYX + ry * HX;
+            unsigned int b[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            b[j] = (8 * g + j < 27) ? (unsigned int)lds[base + off[j]] : 0u;
-        const f32x4 acc = mfma(A, pack8(b), f32x4{0.f, 0.f, 0.f, 0.f});
-        const int gx = x0 + col, gy = y0 + ry, gz = z0 + rz;
-        if (gx < nx && gy < ny && gz < nz)
-            store4<EP>(acc, g, vidx(gx, gy, gz, nx, ny), bias, out16, mask);
+            for (int j = 0; j < 8; ++j)
+                b[j] = (8 * g + j < 27) ? (unsigned int)lds[base + off[j]] : 0u;
+            const f32x4 acc = mfma(A, pack8(b), f32x4{0.f, 0.f, 0.f, 0.f});
+            const int gx = x0 + col, gy = y0 + ry, gz = z0 + rz;
+            if (gx < nx && gy < ny && gz < nz)
+                store4<EP>(acc, g, vidx(gx, gy, gz, nx, ny), bias, out16, mask);
+        }
     }
 }
 
@@ -453,7 +498,7 @@ static_assert(P_END == 14737, "parameter count");
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t a1, a2, a3, dz4, dz3, dz2, dz1, wpart, lpart, total;
+    size_t wpk, a1, a2, a3, dz4, dz3, dz2, dz1, wpart, lpart, total;
     int gx, gy, gz, nconv, nbx, nby, nbricks, wgrid;
 };
 
@@ -463,7 +508,7 @@ Layout layout(int nx, int ny, int nz, bool training)
     const size_t V = (size_t)nx * ny * nz;
     L.gx = (nx + BX - 1) / BX;
     L.gy = (ny + BY - 1) / BY;
-    L.gz = (nz + BZ - 1) / BZ;
+    L.gz = ((nz + BZ - 1) / BZ + ZPB - 1) / ZPB;
     L.nconv = L.gx * L.gy * L.gz;
     L.nbx = (nx + WX - 1) / WX;
     L.nby = (ny + WY - 1) / WY;
@@ -475,6 +520,7 @@ Layout layout(int nx, int ny, int nz, bool training)
         o += align256(bytes);
         return r;
     };
+    L.wpk = take((size_t)WV_COUNT * 14 * 64 * 16);
     L.a1 = take(V * 32);
     L.a2 = take(V * 32);
     if (training) {
@@ -587,14 +633,18 @@ kv_status kv_cnn_forward(kv_context *ctx, const float *d_params, float scale, co
     const dim3 grid(L.gx, L.gy, L.gz);
     hipStream_t s = ctx->stream;
     auto *a1 = at<unsigned short>(d_work, L.a1), *a2 = at<unsigned short>(d_work, L.a2);
+    u32x4 *wpk = at<u32x4>(d_work, L.wpk);
+    auto W = [&](int v) { return wpk + v * 14 * 64; };
+    auto c16 = [](const unsigned short *q) { return reinterpret_cast<const u32x4 *>(q); };
     const float *p = d_params;
-    conv1_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, d_x, scale, p + P_W1, p + P_B1, 16, 1, 0, a1, nullptr);
-    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, reinterpret_cast<const u32x4 *>(a1), p + P_W2, p + P_B2,
-                                                16, 16, 0, a2, nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, nullptr);
-    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, reinterpret_cast<const u32x4 *>(a2), p + P_W3, p + P_B3,
-                                                16, 16, 0, a1, nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, nullptr);
-    conv16_kernel<EP_OUT><<<grid, 256, 0, s>>>(nx, ny, nz, reinterpret_cast<const u32x4 *>(a1), p + P_W4, p + P_B4,
-                                               1, 16, 0, nullptr, nullptr, d_x, nullptr, d_y, scale, 0.f, nullptr);
+    pack_weights_kernel<<<WV_COUNT, 64, 0, s>>>(p, wpk, P_W1, P_W2, P_W3, P_W4);
+    conv1_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, d_x, scale, W(WV_L1), p + P_B1, a1, nullptr);
+    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a1), W(WV_L2), p + P_B2, a2, nullptr, nullptr,
+                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a2), W(WV_L3), p + P_B3, a1, nullptr, nullptr,
+                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_OUT><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a1), W(WV_L4), p + P_B4, nullptr, nullptr, d_x,
+                                               nullptr, d_y, scale, 0.f, nullptr);
     KV_TRY_HIP(hipGetLastError());
     return KV_OK;
 }
@@ -622,26 +672,29 @@ kv_status kv_cnn_gradient(kv_context *ctx, const float *d_params, float scale, c
     const double nvox = (double)nx * ny * nz;
     const float gscale = (float)(2.0 / nvox);
     auto c16 = [](const unsigned short *q) { return reinterpret_cast<const u32x4 *>(q); };
+    u32x4 *wpk = at<u32x4>(d_work, L.wpk);
+    auto W = [&](int v) { return wpk + v * 14 * 64; };
 
-    conv1_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, d_x, scale, p + P_W1, p + P_B1, 16, 1, 0, a1, nullptr);
-    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a1), p + P_W2, p + P_B2, 16, 16, 0, a2, nullptr,
-                                                nullptr, nullptr, nullptr, 0.f, 0.f, nullptr);
-    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a2), p + P_W3, p + P_B3, 16, 16, 0, a3, nullptr,
-                                                nullptr, nullptr, nullptr, 0.f, 0.f, nullptr);
-    conv16_kernel<EP_LOSS><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a3), p + P_W4, p + P_B4, 1, 16, 0, nullptr, nullptr,
-                                                d_x, d_target, dz4, scale, gscale, lpart);
+    pack_weights_kernel<<<WV_COUNT, 64, 0, s>>>(p, wpk, P_W1, P_W2, P_W3, P_W4);
+    conv1_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, d_x, scale, W(WV_L1), p + P_B1, a1, nullptr);
+    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a1), W(WV_L2), p + P_B2, a2, nullptr, nullptr,
+                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a2), W(WV_L3), p + P_B3, a3, nullptr, nullptr,
+                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_LOSS><<<grid, 256, 0, s>>>(nx, ny, nz, c16(a3), W(WV_L4), p + P_B4, nullptr, nullptr, d_x,
+                                                d_target, dz4, scale, gscale, lpart);
     sum_f64_kernel<<<1, 256, 0, s>>>(lpart, L.nconv, lsum);
     // layer 4: weights, then the gradient into a3 (mask a3 > 0)
     KV_TRY_HIP((launch_wgrad<16, 1, true, false>(L, nx, ny, nz, dz4, a3, 1.f, wpart, d_grad + P_W4, s)));
-    conv1_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, dz4, 1.f, p + P_W4, nullptr, 1, 16, 1, dz3, a3);
+    conv1_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, dz4, 1.f, W(WV_L4T), nullptr, dz3, a3);
     // layer 3
     KV_TRY_HIP((launch_wgrad<16, 16, false, false>(L, nx, ny, nz, dz3, a2, 1.f, wpart, d_grad + P_W3, s)));
-    conv16_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, c16(dz3), p + P_W3, nullptr, 16, 16, 1, dz2, a2, nullptr,
-                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, c16(dz3), W(WV_L3T), nullptr, dz2, a2, nullptr, nullptr,
+                                                nullptr, 0.f, 0.f, nullptr);
     // layer 2
     KV_TRY_HIP((launch_wgrad<16, 16, false, false>(L, nx, ny, nz, dz2, a1, 1.f, wpart, d_grad + P_W2, s)));
-    conv16_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, c16(dz2), p + P_W2, nullptr, 16, 16, 1, dz1, a1, nullptr,
-                                                nullptr, nullptr, 0.f, 0.f, nullptr);
+    conv16_kernel<EP_MASK><<<grid, 256, 0, s>>>(nx, ny, nz, c16(dz2), W(WV_L2T), nullptr, dz1, a1, nullptr, nullptr,
+                                                nullptr, 0.f, 0.f, nullptr);
     // layer 1 (input s*x)
     KV_TRY_HIP((launch_wgrad<1, 16, false, true>(L, nx, ny, nz, dz1, d_x, scale, wpart, d_grad + P_W1, s)));
     KV_TRY_HIP(hipGetLastError());
