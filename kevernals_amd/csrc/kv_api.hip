@@ -36,6 +36,7 @@ void free_tables(KvTables &t)
     free_dev(t.lx2);
     free_dev(t.ly2);
     free_dev(t.lz2);
+    free_dev(t.zs);
 }
 
 kv_status dev_alloc(void **p, size_t bytes)
@@ -87,7 +88,7 @@ kv_status ensure_tables(kv_context *ctx, int mode)
         KvTables &t = ctx->tab_r;
         kv_status st;
         if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
-            (st = dev_alloc((void **)&t.z, nze * 4))) {
+            (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4))) {
             free_tables(t);
             return st;
         }
@@ -102,7 +103,7 @@ kv_status ensure_tables(kv_context *ctx, int mode)
     if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
         (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.lx2, (size_t)g.np * g.nu * 4)) ||
         (st = dev_alloc((void **)&t.ly2, (size_t)g.np * g.nv * 4)) ||
-        (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4))) {
+        (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4))) {
         free_tables(t);
         return st;
     }
@@ -246,9 +247,12 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
         return KV_ERR_INVALID;
     }
     // voxel gathers use 32-bit byte offsets (at most 2^30 voxels); ray ids
-    // are 31-bit
+    // are 31-bit; the backprojector addresses one view's plane (two rows of
+    // slack included) with 31-bit byte offsets
     const int64_t nvox = (int64_t)q.nx * q.ny * q.nz, nray = (int64_t)q.np * q.nu * q.nv;
-    if (nvox > (1LL << 30) || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24)) {
+    const int64_t plane_bytes = (int64_t)q.nu * ((int64_t)q.nv + 2) * 4 + 16;
+    if (nvox > (1LL << 30) || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24) ||
+        plane_bytes > 0x7FFFFFFFLL) {
         kv_set_error("problem too large: %lld voxels, %lld rays", (long long)nvox, (long long)nray);
         return KV_ERR_INVALID;
     }

@@ -137,6 +137,40 @@ __global__ void tables_a_kernel(KvGeom g, KvTables t)
     }
 }
 
+// Per view and plane: the (view, slice)-uniform terms of the backprojector,
+// computed with the kernel's own expressions so that T and AZ are the same
+// FP32 values (see KvTables::zs).
+__global__ void zs_kernel(KvGeom g, float *__restrict__ zs)
+{
+    const int n = g.np * (g.nz + 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int l = i % (g.nz + 1), p = i / (g.nz + 1);
+    const float *S = g.src + 3 * p;
+    const float *D = g.det + 3 * p;
+    const float sfx = S[0] / g.sx, sfy = S[1] / g.sy, sfz = S[2] / g.sz;
+    const float ofx = D[0] / g.sx, ofy = D[1] / g.sy, ofz = D[2] / g.sz;
+    const float dd = ofz - sfz;
+    const float cvz = (float)l - (float)g.nz / 2.f;
+    const float T = dd / (cvz - sfz);
+    float az = 0.f;
+    if (l < g.nz) {
+        const float rdz = (cvz + 0.5f) - sfz;  // Projector.cu:473-480
+        az = fabsf(rdz) > 1e-24f ? (ofz - sfz) / rdz : 0.f;
+    }
+    const float scx = g.sx / g.pu, scy = g.sy / g.pv;
+    float *o = zs + (size_t)i * KV_ZS;
+    o[0] = T;
+    o[1] = az;
+    o[2] = T * scx;
+    o[3] = ((sfx - sfx * T) - ofx) * scx - 0.5f;
+    o[4] = T * scy;
+    o[5] = ((sfy - sfy * T) - ofy) * scy - 0.5f;
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
 // ------------------------------------------------------ tiled kernel -----
 constexpr int TX = 64;           // voxels along x per workgroup (one per lane)
 constexpr int TY = 4;            // y rows per workgroup (one per wavefront)
@@ -157,24 +191,25 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fm
 __device__ __forceinline__ float bmin(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
 __device__ __forceinline__ float bmax(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
 
-// Detector index range [lo, hi] (pixel-centre units) whose centres fall in
-// the shadow of [c, c+1] (lateral, floating voxels) between depth factors
-// t0 and t1, widened by `pad` pixels.  Returns false when the shadow is not
-// defined (box not strictly between source and detector plane).
-__device__ __forceinline__ bool shadow(float c, float s, float o, float t0, float t1, float scale, int n, float pad,
-                                       int &lo, int &hi)
+// Detector index range [lo, hi] whose pixel centres can fall in the shadow of
+// the lateral interval [c, c+1] between the depths of two planes, given each
+// plane's affine map u = K0 + c * K1 (K1 > 0) from the per-view table.  The
+// map is increasing in c, so the shadow spans [min over the planes at c, max
+// over the planes at c+1].  The range only has to contain every pixel whose
+// ray crosses the voxel with positive weight (the weight test itself decides
+// acceptance), so it is widened by 0.02 pixel, far above the FP32 rounding
+// of these bounds and of the crossing tables (~1e-3 pixel).
+__device__ __forceinline__ void shadow_affine(float c, float k1a, float k0a, float k1b, float k0b, int n, int &lo,
+                                              int &hi)
 {
-    if (!(t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f))
-        return false;
-    const float e0 = s + (c - s) * t0, e1 = s + ((c + 1.f) - s) * t0;
-    const float e2 = s + (c - s) * t1, e3 = s + ((c + 1.f) - s) * t1;
-    const float mn = fminf(fminf(e0, e1), fminf(e2, e3));
-    const float mx = fmaxf(fmaxf(e0, e1), fmaxf(e2, e3));
-    const float a = clampf((mn - o) * scale - 0.5f - pad, -2.f, (float)n + 1.f);
-    const float b = clampf((mx - o) * scale - 0.5f + pad, -2.f, (float)n + 1.f);
+    const float pad = 0.02f;
+    const float c1 = c + 1.f;
+    const float mn = fminf(__builtin_fmaf(c, k1a, k0a), __builtin_fmaf(c, k1b, k0b));
+    const float mx = fmaxf(__builtin_fmaf(c1, k1a, k0a), __builtin_fmaf(c1, k1b, k0b));
+    const float a = clampf(mn - pad, -2.f, (float)n + 1.f);
+    const float b = clampf(mx + pad, -2.f, (float)n + 1.f);
     lo = max((int)ceilf(a), 0);
     hi = min((int)floorf(b), n - 1);
-    return true;
 }
 
 // LDS of one brick: table bands and the per-voxel running sums.
@@ -185,6 +220,7 @@ struct BpShared {
     float yb[(TY + 1) * YBR];
     int ybase_s;
     float accs[ZT][2][256];  // (value, weight) per slice of the brick
+    float zsb[(ZT + 1) * KV_ZS];  // per-view table rows of planes z0 .. z0 + ZT
 };
 
 // Backprojection of brick (bxi, byi, bzi) of depth zt <= ZT slices.  Called
@@ -213,6 +249,8 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
     const float tol = MODE == KV_MODE_R ? 1e-24f : 0.f;
     const int half = 5;
     const float wx = (float)half * g.pu / g.sx, wy = (float)half * g.pv / g.sy;
+    const float ipu = 1.f / g.pu, ipv = 1.f / g.pv;
+    const bool fast_window = g.nu < (1 << 22) && g.nv < (1 << 22);
 
     for (int i = 0; i < ZT; ++i)
         accs[i][0][tid] = accs[i][1][tid] = 0.f;
@@ -259,6 +297,9 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                 v = tx[(size_t)k * g.nu + px];
             xb[i * XBS + j] = v;
         }
+        const float *zsp = t.zs + ((size_t)p * nz1 + z0) * KV_ZS;
+        for (int e = tid; e < (nzt + 1) * KV_ZS; e += 256)
+            sm.zsb[e] = zsp[e];
         const int ybase = ybase_s;
         for (int e = tid; e < (TY + 1) * YBR; e += 256) {
             const int c = e / YBR, r = e - c * YBR;
@@ -277,6 +318,8 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
         const float efx = (D[0] + (float)g.nu * g.pu) / g.sx;
         const float efy = (D[1] + (float)g.nv * g.pv) / g.sy;
         const float *pr = proj + (size_t)p * g.nu * g.nv;
+        const __amdgpu_buffer_rsrc_t prs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)pr, (short)0, g.nu * g.nv * 4, 0x00020000);
         const float *lx2 = MODE == KV_MODE_A ? t.lx2 + (size_t)p * g.nu : nullptr;
         const float *ly2 = MODE == KV_MODE_A ? t.ly2 + (size_t)p * g.nv : nullptr;
         const float lz2 = MODE == KV_MODE_A ? t.lz2[p] : 0.f;
@@ -285,37 +328,51 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
 
         for (int zi = 0; zi < nzt; ++zi) {
             const int z = z0 + zi;
-            const float cvz = (float)z - hz;
-            int lx = 0, hxw = g.nu - 1, ly = 0, hyw = g.nv - 1;
+            const float *Z = sm.zsb + zi * KV_ZS;  // plane z; Z + KV_ZS is plane z + 1
+            const float t0 = Z[0], t1 = Z[KV_ZS];
+            // shadow of the voxel box [cvx, cvx+1] x [cvy, cvy+1] x [z, z+1]
+            // (conservative detector index ranges, see shadow_affine)
+            const bool has_shadow = t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f;
+            int fx0 = 0, fx1 = g.nu - 1, fy0 = 0, fy1 = g.nv - 1;
+            if (has_shadow) {
+                shadow_affine(cvx, Z[2], Z[3], Z[KV_ZS + 2], Z[KV_ZS + 3], g.nu, fx0, fx1);
+                shadow_affine(cvy, Z[4], Z[5], Z[KV_ZS + 4], Z[KV_ZS + 5], g.nv, fy0, fy1);
+            }
+            int lx = fx0, hxw = fx1, ly = fy0, hyw = fy1;
             if (MODE == KV_MODE_R) {
                 // voxel centre on the detector and the +-5 window,
-                // Projector.cu:473-492, 607-646
-                const float rdz = (cvz + 0.5f) - sfz;
-                const float az = fabsf(rdz) > 1e-24f ? (ofz - sfz) / rdz : 0.f;
+                // Projector.cu:473-492, 607-646 (az is the reference's
+                // (ofz - sfz) / rdz, from the per-view table)
+                const float az = Z[1];
                 const float ix = sfx + rdx * az;
                 const float iy = sfy + rdy * az;
                 if (!(ix > ofx - wx && ix < efx + wx && iy > ofy - wy && iy < efy + wy))
                     continue;
-                const int pcx = (int)((ix - ofx) * g.sx / g.pu);
-                const int pcy = (int)((iy - ofy) * g.sy / g.pv);
-                lx = pcx - half;
-                hxw = pcx + half;
-                ly = pcy - half;
-                hyw = pcy + half;
-                lx = lx < 0 ? 0 : (lx >= g.nu ? g.nu - 1 : lx);
-                hxw = hxw < 0 ? 0 : (hxw >= g.nu ? g.nu - 1 : hxw);
-                ly = ly < 0 ? 0 : (ly >= g.nv ? g.nv - 1 : ly);
-                hyw = hyw < 0 ? 0 : (hyw >= g.nv ? g.nv - 1 : hyw);
-            }
-            // shadow of the voxel box
-            const float t0 = dd / (cvz - sfz), t1 = dd / ((cvz + 1.f) - sfz);
-            int fx0, fx1, fy0, fy1;
-            if (shadow(cvx, sfx, ofx, t0, t1, scx, g.nu, 0.01f, fx0, fx1) &&
-                shadow(cvy, sfy, ofy, t0, t1, scy, g.nv, 0.01f, fy0, fy1)) {
-                lx = max(lx, fx0);
-                hxw = min(hxw, fx1);
-                ly = max(ly, fy0);
-                hyw = min(hyw, fy1);
+                const float qx = (ix - ofx) * g.sx, qy = (iy - ofy) * g.sy;
+                // The window is [pc-5, pc+5] with pc = (int)(q / pitch).  When
+                // the shadow fits inside it for every pc within one of an
+                // estimate, the window changes nothing and the division is
+                // skipped: |q * (1/pitch) - q / pitch| < 1/2 for the
+                // |q / pitch| <= n + 5 < 2^22 that pass the test above, so the
+                // truncations differ by at most one.
+                bool exact = !has_shadow || !fast_window;
+                if (!exact) {
+                    const int ex = (int)(qx * ipu), ey = (int)(qy * ipv);
+                    exact = !(fx0 >= ex - 4 && fx1 <= ex + 4 && fy0 >= ey - 4 && fy1 <= ey + 4);
+                }
+                if (exact) {
+                    const int pcx = (int)(qx / g.pu);
+                    const int pcy = (int)(qy / g.pv);
+                    int wlx = pcx - half, whx = pcx + half, wly = pcy - half, why = pcy + half;
+                    wlx = wlx < 0 ? 0 : (wlx >= g.nu ? g.nu - 1 : wlx);
+                    whx = whx < 0 ? 0 : (whx >= g.nu ? g.nu - 1 : whx);
+                    wly = wly < 0 ? 0 : (wly >= g.nv ? g.nv - 1 : wly);
+                    why = why < 0 ? 0 : (why >= g.nv ? g.nv - 1 : why);
+                    lx = max(wlx, fx0);
+                    hxw = min(whx, fx1);
+                    ly = max(wly, fy0);
+                    hyw = min(why, fy1);
+                }
             }
             if (lx > hxw || ly > hyw)
                 continue;
@@ -331,13 +388,11 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             if (ncol <= 4 && in_band && rows_in_band) {
                 // x intervals of the (at most four) columns stay in registers
                 float Lx[4], Hx[4], L2[4];
-                int cx[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     Lx[c] = 1.f;
                     Hx[c] = 0.f;  // empty: any weight <= -1
                     L2[c] = 0.f;
-                    cx[c] = min(lx + c, g.nu - 1);  // in-bounds address for every slot
                     if (c < ncol) {
                         const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
                         Lx[c] = bmax(lzp, bmin(a1x, a2x));
@@ -346,19 +401,32 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             L2[c] = lx2[lx + c];
                     }
                 }
+                // Projection values through a bounds-checked buffer resource
+                // over this view's plane: the four columns of a row are one
+                // 32-bit offset plus immediate offsets, columns past the row
+                // end read the next row (never used: their weight is
+                // negative) and reads past the plane return 0 instead of
+                // faulting.  The next row's values are requested before the
+                // current row is reduced.
+                float v[4] = {0.f, 0.f, 0.f, 0.f}, vn[4] = {0.f, 0.f, 0.f, 0.f};
+                int off = (ly * g.nu + lx) * 4;
+                if (EP != KV_EP_COUNT && !UNIT) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        v[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
+                }
                 for (int py = ly; py <= hyw; ++py) {
                     const int r = py - ybase;
                     const float a1y = yr0[r], a2y = yr1[r];
                     // max/min are exact: the reference's grouping
                     // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
                     const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
-                    const float *prow = pr + (size_t)py * g.nu;
                     const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
-                    float v[4];
+                    off += 4 * g.nu;
                     if (EP != KV_EP_COUNT && !UNIT) {
 #pragma unroll
                         for (int c = 0; c < 4; ++c)
-                            v[c] = prow[cx[c]];  // issued together; used only if accepted
+                            vn[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
                     }
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
@@ -379,6 +447,9 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                                 a = ok ? a + w * (L * v[c]) : a;
                         }
                     }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        v[c] = vn[c];
                 }
             } else {
                 for (int py = ly; py <= hyw; ++py) {
@@ -530,12 +601,16 @@ int kv_bp_batch_blocks_per_cu()
 hipError_t kv_launch_tables_r(const KvGeom &g, const KvTables &t, hipStream_t s)
 {
     tables_r_kernel<<<2048, 256, 0, s>>>(g, t);
+    const int n = g.np * (g.nz + 1);
+    zs_kernel<<<(n + 255) / 256, 256, 0, s>>>(g, t.zs);
     return hipGetLastError();
 }
 
 hipError_t kv_launch_tables_a(const KvGeom &g, const KvTables &t, hipStream_t s)
 {
     tables_a_kernel<<<2048, 256, 0, s>>>(g, t);
+    const int n = g.np * (g.nz + 1);
+    zs_kernel<<<(n + 255) / 256, 256, 0, s>>>(g, t.zs);
     return hipGetLastError();
 }
 

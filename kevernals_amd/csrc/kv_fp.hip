@@ -74,8 +74,9 @@ __device__ __forceinline__ void axis_range(float d, float s, float h, int n, flo
     hi = mx;
 }
 
-// Mode-R walk of one ray.  visit(voxel_index, weight) is called for every
-// accepted segment (Projector.cu:356-366) in the order of the sorted list.
+// Mode-R walk of one ray.  visit(ok, voxel_index, weight) is called for every
+// segment of the sorted list in order; ok marks the segments the reference
+// accepts (Projector.cu:356-366).
 template <class Visit>
 __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&visit)
 {
@@ -126,27 +127,26 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
 
     // A run's head holds +INF once the run is exhausted, so the next list
-    // entry is the plain minimum of the four heads.  Equal heads are
-    // interchangeable: the sorted sequence of values is unique.
+    // entry is the plain minimum of the four heads.  All heads equal to the
+    // minimum are consumed together: equal entries of the sorted list only
+    // bound zero-width segments, which Projector.cu:358-359 rejects (w must
+    // exceed 1e-6), so the accepted segments and their order are those of
+    // the full list.  The 1.0 fillers are one head for the same reason.
     if (cx == 0) ax = kInf;
     if (cy == 0) ay = kInf;
     if (cz == 0) az = kInf;
     float af = cf > 0 ? 1.0f : kInf;
     auto pop = [&]() -> float {
         const float m = fminf(fminf(ax, ay), fminf(az, af));
-        const bool tx = ax == m;
-        const bool ty = !tx && ay == m;
-        const bool tz = !tx && !ty && az == m;
-        const bool tf = !tx && !ty && !tz;
+        const bool tx = ax == m, ty = ay == m, tz = az == m;
         cx -= tx;
         cy -= ty;
         cz -= tz;
-        cf -= tf;
         // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
         ax = tx ? (cx == 0 ? kInf : ax + ix) : ax;
         ay = ty ? (cy == 0 ? kInf : ay + iy) : ay;
         az = tz ? (cz == 0 ? kInf : az + iz) : az;
-        af = cf == 0 ? kInf : af;
+        af = af == m ? kInf : af;
         return m;
     };
 
@@ -159,17 +159,22 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     const double bz = (double)(r.hz + r.sz);
     const double hdx = 0.5 * (double)r.dx, hdy = 0.5 * (double)r.dy, hdz = 0.5 * (double)r.dz;
     const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz;
+    // visit(ok, k, w) for every segment in list order: ok says whether it
+    // is accepted; k is the voxel index when ok, and 0 (a valid address)
+    // otherwise, so callers may gather unconditionally.
+    (void)n;
     float prev = pop();
-    for (int i = 0; i < n - 1; ++i) {
-        const float cur = pop();
+    float cur = pop();
+    while (cur != kInf) {  // +INF: every run and the fillers consumed
         const float w = cur - prev;
         const double sum = (double)(cur + prev);
         const int vx = (int)__builtin_fma(sum, hdx, bx);
         const int vy = (int)__builtin_fma(sum, hdy, by);
         const int vz = (int)__builtin_fma(sum, hdz, bz);
-        if ((unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol)
-            visit(((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx, w);
+        const bool ok = (unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol;
+        visit(ok, ok ? ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx : 0u, w);
         prev = cur;
+        cur = pop();
     }
 }
 
@@ -199,14 +204,17 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
     if (EP == KV_EP_COUNT) {
         int c = 0;
-        walk_r(g, r, [&](unsigned, float) { ++c; });
+        walk_r(g, r, [&](bool ok, unsigned, float) { c += ok; });
         out[o] = (float)c;
         return;
     }
     float tot = 0.f, tw = 0.f;
-    walk_r(g, r, [&](unsigned k, float w) {
-        tw += w;
-        tot += w * ld_voxel(vol, k);
+    // branch-free accumulation: rejected segments gather voxel 0 and leave
+    // the sums untouched (selects, not additions of zero)
+    walk_r(g, r, [&](bool ok, unsigned k, float w) {
+        const float val = ld_voxel(vol, k);
+        tw = ok ? tw + w : tw;
+        tot = ok ? tot + w * val : tot;
     });
     const float fp = tw > 1e-6f ? tot / tw : 0.f;
     if (EP == KV_EP_PLAIN) {
@@ -443,7 +451,9 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
     const int per = g.nu * g.nv;
     const int p = id / per, rem = id - p * per, v = rem / g.nu, u = rem - v * g.nu;
     int c = 0;
-    auto rec = [&](unsigned k, float wt) {
+    auto rec = [&](bool ok, unsigned k, float wt) {
+        if (!ok)
+            return;
         if (c < cap) {
             idx[(size_t)t * cap + c] = (int64_t)k;
             w[(size_t)t * cap + c] = wt;
@@ -457,7 +467,7 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
     } else {
         RayA r;
         ray_setup_a(g, p, u, v, r);
-        walk_a(r, rec);
+        walk_a(r, [&](unsigned k, float wt) { rec(true, k, wt); });
     }
     counts[t] = c;
 }

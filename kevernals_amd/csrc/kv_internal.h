@@ -33,7 +33,16 @@ struct KvTables {
     float *lx2 = nullptr; // mode A: [np][nu]
     float *ly2 = nullptr; // mode A: [np][nv]
     float *lz2 = nullptr; // mode A: [np]
+    // Per view and z-plane l = 0..nz, KV_ZS floats (both modes):
+    //  [0] T  = (qz - sz) / ((l - nz/2) - sz), the lateral scale of plane l's
+    //          shadow on the detector (Projector.cu:473-480 form)
+    //  [1] AZ = the same ratio at the slice centre l + 0.5 (0 if singular)
+    //  [2] Kx1, [3] Kx0: detector column (pixel-centre units) of lateral x
+    //          coordinate c on plane l, u ~= Kx0 + c * Kx1 (bounds only)
+    //  [4] Ky1, [5] Ky0: the same for detector rows
+    float *zs = nullptr;
 };
+constexpr int KV_ZS = 8;
 
 // Epilogues of the projection kernels: plain result, fused MLEM ratio/update,
 // fused SART residual/update, or the count of accepted (ray, voxel) pairs.
