@@ -74,10 +74,32 @@ __device__ __forceinline__ void axis_range(float d, float s, float h, int n, flo
     hi = mx;
 }
 
+// Minimum of four heads as v_min3_f32 + v_min_f32.  The heads are finite or
+// +INF (never NaN), so the IEEE-mode operand canonicalisation the compiler
+// would add for fminf is not needed.
+__device__ __forceinline__ float min4(float a, float b, float c, float d)
+{
+    float m;
+    asm("v_min3_f32 %0, %1, %2, %3\n\tv_min_f32 %0, %0, %4" : "=&v"(m) : "v"(a), "v"(b), "v"(c), "v"(d));
+    return m;
+}
+
+// Raw buffer resource over the volume (byte range clamped to 32 bits; the
+// volume holds at most 2^30 voxels).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const KvGeom &g, const float *vol)
+{
+    const unsigned long long bytes = (unsigned long long)g.nx * g.ny * g.nz * 4ull;
+    const int n = bytes >= 0xFFFFFFFFull ? (int)0xFFFFFFFFu : (int)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc((void *)vol, (short)0, n, 0x00020000);
+}
+
 // Mode-R walk of one ray.  visit(ok, voxel_index, weight) is called for every
 // segment of the sorted list in order; ok marks the segments the reference
 // accepts (Projector.cu:356-366).
-template <class Visit>
+// U24: nx * ny < 2^24, so an accepted segment's index vz*(nx*ny) + vy*nx + vx
+// is computed with full-rate 24-bit multiplies (a rejected segment's index is
+// never dereferenced outside a bounds-checked buffer access).
+template <bool U24 = false, class Visit>
 __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&visit)
 {
     const float tol = 1e-6f;
@@ -137,7 +159,7 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     if (cz == 0) az = kInf;
     float af = cf > 0 ? 1.0f : kInf;
     auto pop = [&]() -> float {
-        const float m = fminf(fminf(ax, ay), fminf(az, af));
+        const float m = min4(ax, ay, az, af);
         const bool tx = ax == m, ty = ay == m, tz = az == m;
         cx -= tx;
         cy -= ty;
@@ -158,10 +180,10 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     const double by = (double)(r.hy + r.sy);
     const double bz = (double)(r.hz + r.sz);
     const double hdx = 0.5 * (double)r.dx, hdy = 0.5 * (double)r.dy, hdz = 0.5 * (double)r.dz;
-    const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz;
+    const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz, nxy = nx * ny;
     // visit(ok, k, w) for every segment in list order: ok says whether it
-    // is accepted; k is the voxel index when ok, and 0 (a valid address)
-    // otherwise, so callers may gather unconditionally.
+    // is accepted; k is the voxel index when ok and meaningless (possibly
+    // out of range) otherwise.
     (void)n;
     float prev = pop();
     float cur = pop();
@@ -172,7 +194,9 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         const int vy = (int)__builtin_fma(sum, hdy, by);
         const int vz = (int)__builtin_fma(sum, hdz, bz);
         const bool ok = (unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol;
-        visit(ok, ok ? ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx : 0u, w);
+        const unsigned k = U24 ? __umul24((unsigned)vz, nxy) + __umul24((unsigned)vy, nx) + (unsigned)vx
+                               : ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx;
+        visit(ok, k, w);
         prev = cur;
         cur = pop();
     }
@@ -209,13 +233,20 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
         return;
     }
     float tot = 0.f, tw = 0.f;
-    // branch-free accumulation: rejected segments gather voxel 0 and leave
-    // the sums untouched (selects, not additions of zero)
-    walk_r(g, r, [&](bool ok, unsigned k, float w) {
-        const float val = ld_voxel(vol, k);
+    // Branch-free accumulation: every segment gathers through a
+    // bounds-checked buffer resource over the volume (an offset outside it
+    // reads 0, and a rejected segment's value is never used), and rejected
+    // segments leave the sums untouched (selects, not additions of zero).
+    const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
+    auto acc = [&](bool ok, unsigned k, float w) {
+        const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
         tw = ok ? tw + w : tw;
         tot = ok ? tot + w * val : tot;
-    });
+    };
+    if ((long long)g.nx * g.ny < (1LL << 24))
+        walk_r<true>(g, r, acc);
+    else
+        walk_r<false>(g, r, acc);
     const float fp = tw > 1e-6f ? tot / tw : 0.f;
     if (EP == KV_EP_PLAIN) {
         out[o] = fp;
