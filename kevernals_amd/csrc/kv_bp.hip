@@ -522,7 +522,7 @@ constexpr int kZT = 8;
 
 // One brick per workgroup.
 template <int MODE, int EP, int ZT, bool UNIT>
-__global__ __launch_bounds__(256) void bp_tiled_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
+__global__ __launch_bounds__(256, 5) void bp_tiled_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
                                                        float *__restrict__ vol, const float *__restrict__ sens)
 {
     __shared__ BpShared<ZT> sm;
