@@ -249,7 +249,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
     const float tol = MODE == KV_MODE_R ? 1e-24f : 0.f;
     const int half = 5;
     const float wx = (float)half * g.pu / g.sx, wy = (float)half * g.pv / g.sy;
-    const float ipu = 1.f / g.pu, ipv = 1.f / g.pv;
+    const float ipu = 1.f / g.pu;
     const bool fast_window = g.nu < (1 << 22) && g.nv < (1 << 22);
 
     for (int i = 0; i < ZT; ++i)
@@ -310,13 +310,49 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             yb[e] = v;
         }
         __syncthreads();
+
+        const float rdx = (cvx + 0.5f) - sfx, rdy = (cvy + 0.5f) - sfy;
+        const float efx = (D[0] + (float)g.nu * g.pu) / g.sx;
+        const float efy = (D[1] + (float)g.nv * g.pv) / g.sy;
+        // Terms of slice zi that are uniform over the wavefront (its voxels
+        // share y and z), computed once per view by lane zi and read back
+        // with v_readlane: the detector-row range (shadow, intersected with
+        // the +-5 window in mode R, Projector.cu:607-646), whether the voxel
+        // centre's row lies on the widened detector, whether the shadow is
+        // defined, and the z-plane alpha interval.
+        int yr_lo = 0, yr_hi = -1, yflags = 0;
+        float zlz = 0.f, zuz = 0.f, zlzp = 0.f;
+        if (lane < nzt) {
+            const float *Z = sm.zsb + lane * KV_ZS;
+            const float t0 = Z[0], t1 = Z[KV_ZS];
+            const bool hs = t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f;
+            int f0 = 0, f1 = g.nv - 1;
+            if (hs)
+                shadow_affine(cvy, Z[4], Z[5], Z[KV_ZS + 4], Z[KV_ZS + 5], g.nv, f0, f1);
+            bool yin = true;
+            if (MODE == KV_MODE_R) {
+                const float iy = sfy + rdy * Z[1];
+                yin = iy > ofy - wy && iy < efy + wy;
+                const int pcy = (int)(((iy - ofy) * g.sy) / g.pv);
+                int wly = pcy - half, why = pcy + half;
+                wly = wly < 0 ? 0 : (wly >= g.nv ? g.nv - 1 : wly);
+                why = why < 0 ? 0 : (why >= g.nv ? g.nv - 1 : why);
+                f0 = max(f0, wly);
+                f1 = min(f1, why);
+            }
+            yr_lo = f0;
+            yr_hi = f1;
+            yflags = (yin ? 1 : 0) | (hs ? 2 : 0);
+            const int z = z0 + lane;
+            const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
+            zlz = fmaxf(0.f, fminf(a1z, a2z));
+            zuz = fminf(1.f, fmaxf(a1z, a2z));
+            zlzp = bmax(0.f, fminf(a1z, a2z));  // lz with a +0 floor in bit order
+        }
         if (!active)
             continue;
 
         const int xb0 = xbase[lane], xb1 = xbase[lane + 1];
-        const float rdx = (cvx + 0.5f) - sfx, rdy = (cvy + 0.5f) - sfy;
-        const float efx = (D[0] + (float)g.nu * g.pu) / g.sx;
-        const float efy = (D[1] + (float)g.nv * g.pv) / g.sy;
         const float *pr = proj + (size_t)p * g.nu * g.nv;
         const __amdgpu_buffer_rsrc_t prs =
             __builtin_amdgcn_make_buffer_rsrc((void *)pr, (short)0, g.nu * g.nv * 4, 0x00020000);
@@ -327,61 +363,54 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
         const float *yr0 = yb + wv * YBR, *yr1 = yb + (wv + 1) * YBR;
 
         for (int zi = 0; zi < nzt; ++zi) {
-            const int z = z0 + zi;
-            const float *Z = sm.zsb + zi * KV_ZS;  // plane z; Z + KV_ZS is plane z + 1
-            const float t0 = Z[0], t1 = Z[KV_ZS];
-            // shadow of the voxel box [cvx, cvx+1] x [cvy, cvy+1] x [z, z+1]
-            // (conservative detector index ranges, see shadow_affine)
-            const bool has_shadow = t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f;
-            int fx0 = 0, fx1 = g.nu - 1, fy0 = 0, fy1 = g.nv - 1;
-            if (has_shadow) {
+            const int fl = __builtin_amdgcn_readlane(yflags, zi);
+            if (!(fl & 1))
+                continue;  // voxel centre outside the widened detector rows (whole wavefront)
+            const bool has_shadow = (fl & 2) != 0;
+            const float *Z = sm.zsb + zi * KV_ZS;  // plane z0+zi; Z + KV_ZS is the next plane
+            // shadow of the voxel box on the detector columns (conservative,
+            // see shadow_affine); rows come from the per-view pass
+            int fx0 = 0, fx1 = g.nu - 1;
+            if (has_shadow)
                 shadow_affine(cvx, Z[2], Z[3], Z[KV_ZS + 2], Z[KV_ZS + 3], g.nu, fx0, fx1);
-                shadow_affine(cvy, Z[4], Z[5], Z[KV_ZS + 4], Z[KV_ZS + 5], g.nv, fy0, fy1);
-            }
-            int lx = fx0, hxw = fx1, ly = fy0, hyw = fy1;
+            int lx = fx0, hxw = fx1;
+            const int ly = __builtin_amdgcn_readlane(yr_lo, zi), hyw = __builtin_amdgcn_readlane(yr_hi, zi);
             if (MODE == KV_MODE_R) {
                 // voxel centre on the detector and the +-5 window,
                 // Projector.cu:473-492, 607-646 (az is the reference's
                 // (ofz - sfz) / rdz, from the per-view table)
-                const float az = Z[1];
-                const float ix = sfx + rdx * az;
-                const float iy = sfy + rdy * az;
-                if (!(ix > ofx - wx && ix < efx + wx && iy > ofy - wy && iy < efy + wy))
+                const float ix = sfx + rdx * Z[1];
+                if (!(ix > ofx - wx && ix < efx + wx))
                     continue;
-                const float qx = (ix - ofx) * g.sx, qy = (iy - ofy) * g.sy;
-                // The window is [pc-5, pc+5] with pc = (int)(q / pitch).  When
+                const float qx = (ix - ofx) * g.sx;
+                // The window is [pc-5, pc+5] with pc = (int)(qx / pu).  When
                 // the shadow fits inside it for every pc within one of an
                 // estimate, the window changes nothing and the division is
-                // skipped: |q * (1/pitch) - q / pitch| < 1/2 for the
-                // |q / pitch| <= n + 5 < 2^22 that pass the test above, so the
+                // skipped: |qx * (1/pu) - qx / pu| < 1/2 for the
+                // |qx / pu| <= nu + 5 < 2^22 that pass the test above, so the
                 // truncations differ by at most one.
                 bool exact = !has_shadow || !fast_window;
                 if (!exact) {
-                    const int ex = (int)(qx * ipu), ey = (int)(qy * ipv);
-                    exact = !(fx0 >= ex - 4 && fx1 <= ex + 4 && fy0 >= ey - 4 && fy1 <= ey + 4);
+                    const int ex = (int)(qx * ipu);
+                    exact = !(fx0 >= ex - 4 && fx1 <= ex + 4);
                 }
                 if (exact) {
                     const int pcx = (int)(qx / g.pu);
-                    const int pcy = (int)(qy / g.pv);
-                    int wlx = pcx - half, whx = pcx + half, wly = pcy - half, why = pcy + half;
+                    int wlx = pcx - half, whx = pcx + half;
                     wlx = wlx < 0 ? 0 : (wlx >= g.nu ? g.nu - 1 : wlx);
                     whx = whx < 0 ? 0 : (whx >= g.nu ? g.nu - 1 : whx);
-                    wly = wly < 0 ? 0 : (wly >= g.nv ? g.nv - 1 : wly);
-                    why = why < 0 ? 0 : (why >= g.nv ? g.nv - 1 : why);
                     lx = max(wlx, fx0);
                     hxw = min(whx, fx1);
-                    ly = max(wly, fy0);
-                    hyw = min(why, fy1);
                 }
             }
             if (lx > hxw || ly > hyw)
                 continue;
-            const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
-            const float lz = fmaxf(0.f, fminf(a1z, a2z)), uz = fminf(1.f, fmaxf(a1z, a2z));
+            const float lz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlz), zi));
+            const float uz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zuz), zi));
+            const float lzp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlzp), zi));
             // one running sum per voxel across all views, as the reference
             // accumulates (Projector.cu:727-728)
             float a = accs[zi][0][tid], wsm = accs[zi][1][tid];
-            const float lzp = bmax(0.f, fminf(a1z, a2z));  // lz with a +0 floor in bit order
             const int ncol = hxw - lx + 1;
             const bool in_band = lx - xb0 >= 0 && hxw - xb0 < XBW && lx - xb1 >= 0 && hxw - xb1 < XBW;
             const bool rows_in_band = ly - ybase >= 0 && hyw - ybase < YBR;  // wave-uniform
