@@ -8,6 +8,7 @@
 // message (kv_last_error).
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -66,6 +67,26 @@ kv_status check_ctx(kv_context *ctx, bool need_geom)
         return KV_ERR_DEVICE;
     }
     return KV_OK;
+}
+
+// Whether the forward walk may replace its run counters by end thresholds and
+// use 24-bit index multiplies: nx*ny < 2^24, and per axis
+// (n + 2) * (max|source| + n/2 + 2) < 2^22 in voxel units, which keeps the
+// FP32 accumulation error of a run below a quarter of its increment.
+bool walk_fast_ok(const kv_geometry &q)
+{
+    if ((long long)q.nx * q.ny >= (1LL << 24))
+        return false;
+    const int n[3] = {q.nx, q.ny, q.nz};
+    const double sp[3] = {q.sx, q.sy, q.sz};
+    for (int a = 0; a < 3; ++a) {
+        double smax = 0.0;
+        for (int p = 0; p < q.np; ++p)
+            smax = std::max(smax, std::fabs((double)q.src[3 * p + a] / sp[a]));
+        if (((double)n[a] + 2.0) * (smax + (double)n[a] / 2.0 + 2.0) >= 4194304.0)
+            return false;
+    }
+    return true;
 }
 
 kv_status check_mode(int mode)
@@ -279,6 +300,7 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     g.pu = q.pu; g.pv = q.pv;
     g.det = ctx->d_det;
     g.src = ctx->d_src;
+    g.walk_fast = walk_fast_ok(q) ? 1 : 0;
     ctx->has_geom = true;
     return KV_OK;
 }

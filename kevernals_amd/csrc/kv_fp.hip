@@ -74,13 +74,13 @@ __device__ __forceinline__ void axis_range(float d, float s, float h, int n, flo
     hi = mx;
 }
 
-// Minimum of four heads as v_min3_f32 + v_min_f32.  The heads are finite or
+// Minimum of three run heads as one v_min3_f32.  The heads are finite or
 // +INF (never NaN), so the IEEE-mode operand canonicalisation the compiler
 // would add for fminf is not needed.
-__device__ __forceinline__ float min4(float a, float b, float c, float d)
+__device__ __forceinline__ float min3f(float a, float b, float c)
 {
     float m;
-    asm("v_min3_f32 %0, %1, %2, %3\n\tv_min_f32 %0, %0, %4" : "=&v"(m) : "v"(a), "v"(b), "v"(c), "v"(d));
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(a), "v"(b), "v"(c));
     return m;
 }
 
@@ -96,10 +96,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const KvGeom &g, c
 // Mode-R walk of one ray.  visit(ok, voxel_index, weight) is called for every
 // segment of the sorted list in order; ok marks the segments the reference
 // accepts (Projector.cu:356-366).
-// U24: nx * ny < 2^24, so an accepted segment's index vz*(nx*ny) + vy*nx + vx
-// is computed with full-rate 24-bit multiplies (a rejected segment's index is
-// never dereferenced outside a bounds-checked buffer access).
-template <bool U24 = false, class Visit>
+// FAST (the geometry's walk_fast flag, set by kv_set_geometry): nx * ny <
+// 2^24, so an accepted segment's index vz*(nx*ny) + vy*nx + vx uses full-rate
+// 24-bit multiplies (a rejected segment's index is never dereferenced
+// outside a bounds-checked buffer access); and for every axis
+// (n + 2) * (max|source| + n/2 + 2) < 2^22 in voxel units.  A run's values are
+// a0 + j*inc accumulated in FP32 with |a| <= (|s| + n/2) * inc, so after j <= n+1
+// additions the accumulated rounding error stays below inc/4: the value
+// T = a0 + (c - 1/2) * inc (FP64, rounded once to FP32) then lies strictly
+// between the run's last value and the one after it, and "exhausted" is one
+// comparison with T instead of a count.  Without FAST each run counts down.
+template <bool FAST, class Visit>
 __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&visit)
 {
     const float tol = 1e-6f;
@@ -121,8 +128,8 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     axis_range(r.dz, r.sz, r.hz, g.nz, amin, amax, loz, hiz);
     const int n = (hix - lox) + (hiy - loy) + (hiz - loz) + 3;
 
-    // Run heads, remaining counts and increments (Projector.cu:139-201).
-    // A run whose axis fails |d| > tol leaves its slots at the 1.0 filler.
+    // Run heads, lengths and increments (Projector.cu:139-201).  A run whose
+    // axis fails |d| > tol leaves its slots at the 1.0 filler.
     int cx = 0, cy = 0, cz = 0;
     float ax = 0.f, ay = 0.f, az = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
     if (okx) {
@@ -146,59 +153,96 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         const int dzr = hiz - loz;
         cz = dzr >= 2 ? dzr - 1 : 1;
     }
-    int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
+    const int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
+    const float tx = FAST ? (float)((double)ax + ((double)cx - 0.5) * (double)ix) : 0.f;
+    const float ty = FAST ? (float)((double)ay + ((double)cy - 0.5) * (double)iy) : 0.f;
+    const float tz = FAST ? (float)((double)az + ((double)cz - 0.5) * (double)iz) : 0.f;
 
     // A run's head holds +INF once the run is exhausted, so the next list
-    // entry is the plain minimum of the four heads.  All heads equal to the
+    // entry is the plain minimum of the heads.  All heads equal to the
     // minimum are consumed together: equal entries of the sorted list only
     // bound zero-width segments, which Projector.cu:358-359 rejects (w must
     // exceed 1e-6), so the accepted segments and their order are those of
-    // the full list.  The 1.0 fillers are one head for the same reason.
+    // the full list.  The 1.0 fillers form one entry for the same reason,
+    // merged in between two loops over the runs (values below 1.0, then the
+    // rare values above it).
     if (cx == 0) ax = kInf;
     if (cy == 0) ay = kInf;
     if (cz == 0) az = kInf;
-    float af = cf > 0 ? 1.0f : kInf;
-    auto pop = [&]() -> float {
-        const float m = min4(ax, ay, az, af);
-        const bool tx = ax == m, ty = ay == m, tz = az == m;
-        cx -= tx;
-        cy -= ty;
-        cz -= tz;
-        // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
-        ax = tx ? (cx == 0 ? kInf : ax + ix) : ax;
-        ay = ty ? (cy == 0 ? kInf : ay + iy) : ay;
-        az = tz ? (cz == 0 ? kInf : az + iz) : az;
-        af = af == m ? kInf : af;
-        return m;
+    // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
+    auto advance = [&](float m) {
+        const bool px = ax == m, py = ay == m, pz = az == m;
+        if (FAST) {
+            const float nx_ = ax + ix, ny_ = ay + iy, nz_ = az + iz;
+            ax = px ? (nx_ > tx ? kInf : nx_) : ax;
+            ay = py ? (ny_ > ty ? kInf : ny_) : ay;
+            az = pz ? (nz_ > tz ? kInf : nz_) : az;
+        } else {
+            cx -= px;
+            cy -= py;
+            cz -= pz;
+            ax = px ? (cx == 0 ? kInf : ax + ix) : ax;
+            ay = py ? (cy == 0 ? kInf : ay + iy) : ay;
+            az = pz ? (cz == 0 ? kInf : az + iz) : az;
+        }
     };
 
-    // Segment loop, Projector.cu:348-367.  The midpoint is the FP32 sum
-    // divided by 2.0 in FP64.  (double)float * (double)float is exact, and
-    // so is the halving, so each index expression equals one FP64 fma of
-    // the FP32 sum with the halved direction.
+    // Segment (prev, cur), Projector.cu:348-367.  The midpoint is the FP32
+    // sum divided by 2.0 in FP64.  (double)float * (double)float is exact,
+    // and so is the halving, so each index expression equals one FP64 fma of
+    // the FP32 sum with the halved direction.  visit(ok, k, w): ok marks an
+    // accepted segment; k is its voxel index when ok and meaningless
+    // (possibly out of range) otherwise.
     const double bx = (double)(r.hx + r.sx);
     const double by = (double)(r.hy + r.sy);
     const double bz = (double)(r.hz + r.sz);
     const double hdx = 0.5 * (double)r.dx, hdy = 0.5 * (double)r.dy, hdz = 0.5 * (double)r.dz;
     const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny, nz = (unsigned)g.nz, nxy = nx * ny;
-    // visit(ok, k, w) for every segment in list order: ok says whether it
-    // is accepted; k is the voxel index when ok and meaningless (possibly
-    // out of range) otherwise.
-    (void)n;
-    float prev = pop();
-    float cur = pop();
-    while (cur != kInf) {  // +INF: every run and the fillers consumed
+    auto segment = [&](float prev, float cur) {
         const float w = cur - prev;
         const double sum = (double)(cur + prev);
         const int vx = (int)__builtin_fma(sum, hdx, bx);
         const int vy = (int)__builtin_fma(sum, hdy, by);
         const int vz = (int)__builtin_fma(sum, hdz, bz);
         const bool ok = (unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol;
-        const unsigned k = U24 ? __umul24((unsigned)vz, nxy) + __umul24((unsigned)vy, nx) + (unsigned)vx
-                               : ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx;
+        const unsigned k = FAST ? __umul24((unsigned)vz, nxy) + __umul24((unsigned)vy, nx) + (unsigned)vx
+                                : ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx;
         visit(ok, k, w);
+    };
+
+    const float fill = cf > 0 ? 1.0f : kInf;
+    // first entry
+    float prev = min3f(ax, ay, az);
+    bool filler_left = cf > 0;
+    if (fill <= prev) {
+        prev = fill;
+        filler_left = false;
+    }
+    advance(prev);
+    // entries below the fillers' value (all of them when there is no filler)
+    const float lim = filler_left ? fill : kInf;
+    for (;;) {
+        const float cur = min3f(ax, ay, az);
+        if (!(cur < lim))
+            break;
+        advance(cur);
+        segment(prev, cur);
         prev = cur;
-        cur = pop();
+    }
+    if (filler_left) {
+        // the 1.0 entry (run values equal to it are the same entry), then
+        // run values above it
+        advance(fill);
+        segment(prev, fill);
+        prev = fill;
+        for (;;) {
+            const float cur = min3f(ax, ay, az);
+            if (cur == kInf)
+                break;
+            advance(cur);
+            segment(prev, cur);
+            prev = cur;
+        }
     }
 }
 
@@ -228,7 +272,7 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
     if (EP == KV_EP_COUNT) {
         int c = 0;
-        walk_r(g, r, [&](bool ok, unsigned, float) { c += ok; });
+        walk_r<false>(g, r, [&](bool ok, unsigned, float) { c += ok; });
         out[o] = (float)c;
         return;
     }
@@ -243,7 +287,7 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
         tw = ok ? tw + w : tw;
         tot = ok ? tot + w * val : tot;
     };
-    if ((long long)g.nx * g.ny < (1LL << 24))
+    if (g.walk_fast)
         walk_r<true>(g, r, acc);
     else
         walk_r<false>(g, r, acc);
@@ -494,7 +538,7 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
     if (mode == KV_MODE_R) {
         RayF r;
         ray_setup(g, p, u, v, r);
-        walk_r(g, r, rec);
+        walk_r<false>(g, r, rec);
     } else {
         RayA r;
         ray_setup_a(g, p, u, v, r);

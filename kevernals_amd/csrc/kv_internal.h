@@ -18,6 +18,7 @@ struct KvGeom {
     float pu, pv;
     const float *det;
     const float *src;
+    int walk_fast;  // forward walk may use end thresholds and 24-bit indices (see kv_fp.hip)
 };
 
 // Per-view plane-crossing tables.  For mode R an entry is the reference's

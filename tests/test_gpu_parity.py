@@ -188,6 +188,38 @@ def test_tiny_geometries_edge_cases(K, O):
         np.testing.assert_allclose(host(bp), o.bp_a(y), rtol=1e-5, atol=1e-6)
 
 
+def walk_fast_condition(g):
+    """Host mirror of the library's choice between the forward walk with end
+    thresholds and 24-bit indices and the counting walk (kv_api.hip
+    walk_fast_ok)."""
+    if g.nx * g.ny >= 1 << 24:
+        return False
+    for a, (n, sp) in enumerate(((g.nx, g.sx), (g.ny, g.sy), (g.nz, g.sz))):
+        smax = float(np.max(np.abs(g.src[:, a].astype(np.float64) / sp)))
+        if (n + 2.0) * (smax + n / 2.0 + 2.0) >= 4194304.0:
+            return False
+    return True
+
+
+def test_both_forward_walks_match_oracle(K, O):
+    """The threshold walk (the workloads' geometries) and the counting walk
+    (a geometry outside the threshold bound: 4096 voxels along x) both give
+    the reference's ray sums bit for bit."""
+    g_fast = tiny_geometry(nviews=3, vol=(37, 11, 5), det=(40, 16))
+    g_slow = tiny_geometry(nviews=3, vol=(4096, 3, 3), det=(48, 6), spacing=(0.05, 1.0, 2.0), pitch=(5.0, 0.6),
+                           sod=300.0, sid=500.0)
+    assert walk_fast_condition(g_fast) and not walk_fast_condition(g_slow)
+    for g in (g_fast, g_slow):
+        o = O.Oracle(g)
+        x = (0.1 + rng(27).random(g.vol_shape)).astype(np.float32)
+        ctx = K.Context(0, g)
+        fp = torch.empty(g.proj_shape, device="cuda")
+        ctx.forward_project(dev(x), fp, K.MODE_R)
+        ref = o.fp_r(x)
+        assert np.count_nonzero(ref) > g.nrays // 4
+        np.testing.assert_array_equal(host(fp), ref)
+
+
 # ------------------------------------------------------------ mode A -------
 def test_mode_a_parity_and_adjoint(K, O):
     g = half_small()
