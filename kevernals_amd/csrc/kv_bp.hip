@@ -415,71 +415,85 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             const bool in_band = lx - xb0 >= 0 && hxw - xb0 < XBW && lx - xb1 >= 0 && hxw - xb1 < XBW;
             const bool rows_in_band = ly - ybase >= 0 && hyw - ybase < YBR;  // wave-uniform
             if (ncol <= 4 && in_band && rows_in_band) {
-                // x intervals of the (at most four) columns stay in registers
-                float Lx[4], Hx[4], L2[4];
+                // x intervals of the (at most NC) columns stay in registers;
+                // NC is the wavefront's widest column range (2, 3 or 4), so
+                // narrow shadows do not pay for four columns per row
+                auto rows = [&](auto nc_tag) {
+                    constexpr int NC = decltype(nc_tag)::value;
+                    float Lx[NC], Hx[NC], L2[NC];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    Lx[c] = 1.f;
-                    Hx[c] = 0.f;  // empty: any weight <= -1
-                    L2[c] = 0.f;
-                    if (c < ncol) {
-                        const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
-                        Lx[c] = bmax(lzp, bmin(a1x, a2x));
-                        Hx[c] = bmin(uz, bmax(a1x, a2x));
-                        if (MODE == KV_MODE_A)
-                            L2[c] = lx2[lx + c];
-                    }
-                }
-                // Projection values through a bounds-checked buffer resource
-                // over this view's plane: the four columns of a row are one
-                // 32-bit offset plus immediate offsets, columns past the row
-                // end read the next row (never used: their weight is
-                // negative) and reads past the plane return 0 instead of
-                // faulting.  The next row's values are requested before the
-                // current row is reduced.
-                float v[4] = {0.f, 0.f, 0.f, 0.f}, vn[4] = {0.f, 0.f, 0.f, 0.f};
-                int off = (ly * g.nu + lx) * 4;
-                if (EP != KV_EP_COUNT && !UNIT) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        v[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
-                }
-                for (int py = ly; py <= hyw; ++py) {
-                    const int r = py - ybase;
-                    const float a1y = yr0[r], a2y = yr1[r];
-                    // max/min are exact: the reference's grouping
-                    // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
-                    const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
-                    const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
-                    off += 4 * g.nu;
-                    if (EP != KV_EP_COUNT && !UNIT) {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            vn[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
-                    }
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
-                        const bool ok = w > tol;
-                        // selects, not adds of zero: skipped pixels leave the
-                        // running sums bit-identical
-                        if (EP == KV_EP_COUNT) {
-                            wsm = ok ? wsm + 1.f : wsm;
-                        } else if (MODE == KV_MODE_R) {
-                            wsm = ok ? wsm + w : wsm;
-                            a = ok ? a + w * v[c] : a;
-                        } else {
-                            const float L = sqrtf((L2[c] + lyy) + lz2);
-                            if (UNIT)
-                                a = ok ? a + w * L : a;
-                            else
-                                a = ok ? a + w * (L * v[c]) : a;
+                    for (int c = 0; c < NC; ++c) {
+                        Lx[c] = 1.f;
+                        Hx[c] = 0.f;  // empty: any weight <= -1
+                        L2[c] = 0.f;
+                        if (c < ncol) {
+                            const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
+                            Lx[c] = bmax(lzp, bmin(a1x, a2x));
+                            Hx[c] = bmin(uz, bmax(a1x, a2x));
+                            if (MODE == KV_MODE_A)
+                                L2[c] = lx2[lx + c];
                         }
                     }
+                    // Projection values through a bounds-checked buffer
+                    // resource over this view's plane: the columns of a row
+                    // are one 32-bit offset plus immediate offsets, columns
+                    // past the row end read the next row (never used: their
+                    // weight is negative) and reads past the plane return 0
+                    // instead of faulting.  The next row's values are
+                    // requested before the current row is reduced.
+                    float v[NC], vn[NC];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        v[c] = vn[c];
-                }
+                    for (int c = 0; c < NC; ++c)
+                        v[c] = vn[c] = 0.f;
+                    int off = (ly * g.nu + lx) * 4;
+                    if (EP != KV_EP_COUNT && !UNIT) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+                            v[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
+                    }
+                    for (int py = ly; py <= hyw; ++py) {
+                        const int r = py - ybase;
+                        const float a1y = yr0[r], a2y = yr1[r];
+                        // max/min are exact: the reference's grouping
+                        // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
+                        const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
+                        const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
+                        off += 4 * g.nu;
+                        if (EP != KV_EP_COUNT && !UNIT) {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                vn[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
+                        }
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
+                            const bool ok = w > tol;
+                            // selects, not adds of zero: skipped pixels leave
+                            // the running sums bit-identical
+                            if (EP == KV_EP_COUNT) {
+                                wsm = ok ? wsm + 1.f : wsm;
+                            } else if (MODE == KV_MODE_R) {
+                                wsm = ok ? wsm + w : wsm;
+                                a = ok ? a + w * v[c] : a;
+                            } else {
+                                const float L = sqrtf((L2[c] + lyy) + lz2);
+                                if (UNIT)
+                                    a = ok ? a + w * L : a;
+                                else
+                                    a = ok ? a + w * (L * v[c]) : a;
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+                            v[c] = vn[c];
+                    }
+                };
+                if (__all(ncol <= 2))
+                    rows(std::integral_constant<int, 2>{});
+                else if (__all(ncol <= 3))
+                    rows(std::integral_constant<int, 3>{});
+                else
+                    rows(std::integral_constant<int, 4>{});
             } else {
                 for (int py = ly; py <= hyw; ++py) {
                     const int r = py - ybase;
