@@ -2,6 +2,8 @@
 itself: 5-iteration MLEM (input) and 100-iteration MLEM (target) of seeded
 phantoms, 256x256x32 voxels at 0.25x0.25x2 mm, 512^2 detector at 0.3 mm,
 15 views on +-15 deg, transmission data at I0 = 1e4.  L2 loss, Adam.
+--stages 5,10,20,40 adds the same run's later estimates as inputs (each
+paired with its 100-iteration estimate).
 
     python -m kevernals_amd.train_cnn --pairs 2000 --epochs 4 --out weights.npz
 
@@ -33,7 +35,10 @@ def log(msg):
     print(msg, flush=True)
 
 
-def make_pairs(ctx, seeds, pool, i0, k_in, k_ref, gen):
+def make_pairs(ctx, seeds, pool, i0, stages, k_ref, gen):
+    """(input, target) pairs: the MLEM-A estimate after each iteration count
+    in `stages`, paired with the estimate after k_ref iterations of the same
+    run (the target tensor is shared by a phantom's pairs)."""
     import torch
 
     from kevernals_amd.learned import transmission_data
@@ -45,12 +50,18 @@ def make_pairs(ctx, seeds, pool, i0, k_in, k_ref, gen):
         mu = torch.from_numpy(mu_h).cuda()
         data = transmission_data(ctx, mu, i0, gen)
         x = torch.ones(g.vol_shape, device="cuda")
-        for _ in range(k_in):
+        snaps = []
+        done = 0
+        for k in stages:
+            for _ in range(k - done):
+                ctx.mlem_iteration(x, data, 1.0, 1)
+            done = k
+            snaps.append(x.clone())
+        for _ in range(k_ref - done):
             ctx.mlem_iteration(x, data, 1.0, 1)
-        xs.append(x.clone())
-        for _ in range(k_ref - k_in):
-            ctx.mlem_iteration(x, data, 1.0, 1)
-        ts.append(x)
+        for sn in snaps:
+            xs.append(sn)
+            ts.append(x)
         if (n + 1) % 50 == 0 or n + 1 == len(seeds):
             torch.cuda.synchronize()
             log(f"[train] pairs {n + 1}/{len(seeds)} in {time.time() - t0:.1f} s")
@@ -65,6 +76,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--i0", type=float, default=1e4)
     ap.add_argument("--k-in", type=int, default=5)
+    ap.add_argument("--stages", default=None,
+                    help="comma-separated input iteration counts (default: --k-in only)")
     ap.add_argument("--k-ref", type=int, default=100)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
@@ -85,14 +98,16 @@ def main(argv=None):
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed + 12345)
     t0 = time.time()
-    xs, ts = make_pairs(ctx, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs)), pool, args.i0, args.k_in,
+    stages = sorted({int(v) for v in args.stages.split(",")}) if args.stages else [args.k_in]
+    assert 0 < stages[0] and stages[-1] < args.k_ref
+    xs, ts = make_pairs(ctx, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs)), pool, args.i0, stages,
                         args.k_ref, gen)
-    vx, vt = make_pairs(ctx, list(range(VAL_SEED0, VAL_SEED0 + args.val)), pool, args.i0, args.k_in, args.k_ref,
+    vx, vt = make_pairs(ctx, list(range(VAL_SEED0, VAL_SEED0 + args.val)), pool, args.i0, [args.k_in], args.k_ref,
                         gen)
     pool.close()
     pool.join()
     t_data = time.time() - t0
-    log(f"[train] {args.pairs} + {args.val} pairs ready in {t_data:.1f} s")
+    log(f"[train] {len(xs)} training pairs (stages {stages}) + {len(vx)} validation pairs in {t_data:.1f} s")
 
     cnn = Cnn(ctx, seed=args.seed)
 
@@ -117,8 +132,9 @@ def main(argv=None):
         history.append((ep + 1, float(np.mean(tl)), vl))
         log(f"[train] epoch {ep + 1}: train {np.mean(tl):.5e} val {vl:.5e} ({time.time() - t1:.1f} s)")
     steps_per_s = cnn.step / (time.time() - t1)
-    cnn.save(args.out, history=np.array(history), identity_val=base, pairs=args.pairs, k_in=args.k_in,
-             k_ref=args.k_ref, i0=args.i0, data_seconds=t_data, steps_per_s=steps_per_s)
+    cnn.save(args.out, history=np.array(history), identity_val=base, pairs=len(xs), phantoms=args.pairs,
+             k_in=args.k_in, stages=np.array(stages), k_ref=args.k_ref, i0=args.i0, data_seconds=t_data,
+             steps_per_s=steps_per_s)
     log(f"[train] saved {args.out}: {cnn.step} steps, {steps_per_s:.1f} steps/s")
     return 0
 
