@@ -269,9 +269,11 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     }
     // voxel gathers use 32-bit byte offsets (at most 2^30 voxels); ray ids
     // are 31-bit; the backprojector addresses one view's plane (two rows of
-    // slack included) with 31-bit byte offsets
+    // slack included) and one view's x and y crossing tables with 31-bit byte
+    // offsets
     const int64_t nvox = (int64_t)q.nx * q.ny * q.nz, nray = (int64_t)q.np * q.nu * q.nv;
-    const int64_t plane_bytes = (int64_t)q.nu * ((int64_t)q.nv + 2) * 4 + 16;
+    const int64_t plane_bytes = std::max({(int64_t)q.nu * ((int64_t)q.nv + 2) * 4 + 16,
+                                          ((int64_t)q.nx + 1) * q.nu * 4, ((int64_t)q.ny + 1) * q.nv * 4});
     if (nvox > (1LL << 30) || nray > 0x7FFFFFFFLL || q.nx > (1 << 24) || q.ny > (1 << 24) || q.nz > (1 << 24) ||
         plane_bytes > 0x7FFFFFFFLL) {
         kv_set_error("problem too large: %lld voxels, %lld rays", (long long)nvox, (long long)nray);

@@ -289,13 +289,20 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             ybase_s = (int)floorf(v) - 2;
         }
         __syncthreads();
+        // Table bands through bounds-checked buffer resources over this
+        // view's tables: entries past a table read 0, and entries whose
+        // detector column or row lies off the detector are never used (the
+        // band slots read below always belong to columns in [0, nu) and rows
+        // in [0, nv)), so no range test is needed.
+        const __amdgpu_buffer_rsrc_t txs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)tx, (short)0, nx1 * g.nu * 4, 0x00020000);
+        const __amdgpu_buffer_rsrc_t tys =
+            __builtin_amdgcn_make_buffer_rsrc((void *)ty, (short)0, ny1 * g.nv * 4, 0x00020000);
         for (int e = tid; e < (TX + 1) * XBW; e += 256) {
             const int i = e / XBW, j = e - i * XBW;
-            const int k = x0 + i, px = xbase[i] + j;
-            float v = 0.f;
-            if (k <= g.nx && px >= 0 && px < g.nu)
-                v = tx[(size_t)k * g.nu + px];
-            xb[i * XBS + j] = v;
+            const unsigned k = (unsigned)(x0 + i), px = (unsigned)(xbase[i] + j);
+            xb[i * XBS + j] = __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(txs, (int)((k * (unsigned)g.nu + px) * 4u), 0, 0));
         }
         const float *zsp = t.zs + ((size_t)p * nz1 + z0) * KV_ZS;
         for (int e = tid; e < (nzt + 1) * KV_ZS; e += 256)
@@ -303,11 +310,9 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
         const int ybase = ybase_s;
         for (int e = tid; e < (TY + 1) * YBR; e += 256) {
             const int c = e / YBR, r = e - c * YBR;
-            const int j = y0 + c, py = ybase + r;
-            float v = 0.f;
-            if (j <= g.ny && py >= 0 && py < g.nv)
-                v = ty[(size_t)j * g.nv + py];
-            yb[e] = v;
+            const unsigned j = (unsigned)(y0 + c), py = (unsigned)(ybase + r);
+            yb[e] = __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(tys, (int)((j * (unsigned)g.nv + py) * 4u), 0, 0));
         }
         __syncthreads();
 
