@@ -235,6 +235,91 @@ def bench_cnn(args, torch, K, G):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned_batched(args, torch, K, G):
+    """Learned plus batched at full size: `--recons` reconstructions of
+    1024x1024x64 / 25 views (phantom seeds 5000+i with 200
+    microcalcifications, I0 = 5e3) in one mode-A batch (persistent launches,
+    hipGraph replay), plain batched MLEM-A vs the same batch with the
+    likelihood-gated CNN (after iteration 5, then every 2 iterations).  Time
+    to every reconstruction's RMSE <= threshold against its own 100-iteration
+    MLEM-A result; times exclude the RMSE evaluations."""
+    from kevernals_amd.learned import batched_gated_mlem, transmission_data
+    from kevernals_amd.phantom import phantom
+
+    g = G.full_plain()
+    n = args.recons
+    t = time.time()
+    b = K.Batch([g] * n, mode=K.MODE_A)
+    gen = torch.Generator(device="cuda")
+    for i in range(n):
+        ctx = b.context(i)
+        gen.manual_seed(5000 + i)
+        mu = torch.from_numpy(phantom(g, 5000 + i, microcalcifications=200)).cuda()
+        data = transmission_data(ctx, mu, 5e3, gen)
+        ctx.copy(b.buffers(i)[1], data, data.numel() * 4)
+        torch.cuda.synchronize()
+        del mu, data
+    ones = np.ones(g.vol_shape, np.float32)
+
+    def reset():
+        for i in range(n):
+            b.set_volume(i, ones)
+
+    reset()
+    log(f"[bench] {n} reconstructions ready in {time.time() - t:.1f} s")
+    t = time.perf_counter()
+    b.mlem(100, use_graph=True)
+    t_ref = time.perf_counter() - t
+    refs = []
+    for i in range(n):
+        r = torch.empty(g.vol_shape, device="cuda")
+        b.context(i).copy(r, b.buffers(i)[0], r.numel() * 4)
+        refs.append(r)
+    torch.cuda.synchronize()
+    log(f"[bench] references: 100 batched MLEM-A iterations in {t_ref:.2f} s")
+    wpath = os.path.join(ROOT, "kevernals_amd", "weights", "cnn_mlem5_to_100.npz")
+    with np.load(wpath, allow_pickle=False) as f:
+        params, scale = f["params"], float(f["scale"])
+    runs = {}
+    for name, prm in (("plain", None), ("learned", params)):
+        reset()
+        runs[name] = batched_gated_mlem(b, 100, params=prm, scale=scale, every=2, first=5, patience=2,
+                                        references=refs)
+
+    def time_all(run, thr):
+        for kind, k, tt, r in run.trace:
+            if r is not None and max(r) <= thr:
+                return tt, k
+        return None, None
+
+    out_t = {}
+    for thr in (0.10, 0.05, 0.02, 0.01):
+        (tp, kp), (tl, kl) = time_all(runs["plain"], thr), time_all(runs["learned"], thr)
+        out_t[f"{thr:g}"] = {"plain_s": tp, "plain_iters": kp, "learned_s": tl, "learned_iters": kl,
+                             "speedup": (tp / tl) if (tp is not None and tl) else None}
+        log(f"[bench] all {n} at RMSE <= {thr:g}: plain {tp} s ({kp} it), learned {tl} s ({kl} it)")
+    s1 = out_t["0.01"]
+    out = {
+        "metric": METRIC,
+        "value": round(s1["speedup"], 3) if s1["speedup"] else None,
+        "unit": "x (plain batched time / learned batched time to RMSE <= 1%)",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * (s1["learned_s"] or runs["learned"].trace[-1][2]), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (MLEM), bf16/f32-accumulate (CNN)",
+        "data": "synthetic (seeded phantoms + 200 microcalcifications, Poisson I0=5e3)",
+        "config": {"workload": "learned plus batched MLEM-A, time to RMSE<=1% vs 100-iteration MLEM",
+                   "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews, "cnn_every": 2,
+                   "cnn_first": 5, "parallelism": "single GPU"},
+        "time_to_rmse": out_t,
+        "cnn": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
+        "reference_seconds": round(t_ref, 3),
+        "final_rmse": {"plain": runs["plain"].trace[-1][3], "learned": runs["learned"].trace[-1][3]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned(args, torch, K, G):
     """Time to RMSE <= 1% against the 100-iteration MLEM result, plain MLEM
     vs MLEM with the likelihood-gated CNN (candidate after iteration 5 and
@@ -320,7 +405,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "cnn"], default="full")
+    ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "learned-batched", "cnn"],
+                    default="full")
+    ap.add_argument("--recons", type=int, default=8, help="reconstructions of the learned-batched workload")
     ap.add_argument("--tile-cache", default=None, help="autotuner cache file (batched workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an elementwise copy of 256 MiB (known bytes) to scale HBM counters")
@@ -341,6 +428,8 @@ def main():
         return bench_learned(args, torch, K, G)
     if args.workload == "cnn":
         return bench_cnn(args, torch, K, G)
+    if args.workload == "learned-batched":
+        return bench_learned_batched(args, torch, K, G)
     g = G.full_plain() if args.workload == "full" else G.small_plain()
     ctx = K.Context(0, g)
     ctx.use_torch_stream()

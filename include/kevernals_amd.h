@@ -87,6 +87,8 @@ kv_status kv_free(kv_context *ctx, void *d_ptr);
 kv_status kv_memcpy_h2d(kv_context *ctx, void *d_dst, const void *h_src, size_t bytes);
 kv_status kv_memcpy_d2h(kv_context *ctx, void *h_dst, const void *d_src, size_t bytes);
 kv_status kv_memset(kv_context *ctx, void *d_dst, int value, size_t bytes);
+/* Device-to-device copy on the context's stream (asynchronous). */
+kv_status kv_memcpy_d2d(kv_context *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* ---- operators (device pointers, asynchronous) --------------------------- */
 /* Forward projection: CudaPerformProjection (Projector.cu:227-372) for
@@ -150,11 +152,12 @@ kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *
 /* ---- batched reconstructions (no reference counterpart) ------------------
  * Many reconstructions of uneven volume and projection-count sizes share one
  * persistent forward launch and one persistent backprojection launch per
- * iteration (mode R, fused MLEM: the per-reconstruction result is bit-identical
- * to kv_mlem_iteration on its own); the iterations can be captured once into
- * a hipGraph and replayed.  The batch owns its device buffers and stream. */
+ * iteration (fused MLEM in mode R -- the reference's recons::MLEM -- or mode A,
+ * x <- x/s * A^T(data/Ax); the per-reconstruction result is bit-identical to
+ * kv_mlem_iteration on its own); the iterations can be captured once into a
+ * hipGraph and replayed.  The batch owns its device buffers and stream. */
 typedef struct kv_batch kv_batch;
-kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch **out);
+kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, int mode, kv_batch **out);
 kv_status kv_batch_destroy(kv_batch *b);
 /* Host <-> device copies of reconstruction i's volume (initial estimate in,
  * result out) and data (projections in). */
@@ -171,6 +174,13 @@ kv_status kv_batch_mlem(kv_batch *b, const int32_t *iters, float relax, int use_
  * per wavefront), backprojection brick depth 4 or 8 slices.  Results do not
  * depend on them. */
 kv_status kv_batch_set_tiles(kv_batch *b, int i, int fp_layout, int bp_slices);
+/* Reconstruction i's context (owned by the batch: do not destroy it; its
+ * geometry and tables are those of the batch) and device buffers, for
+ * per-reconstruction work between batch calls (likelihood, learned step).
+ * Work enqueued through the context must be complete (kv_synchronize) before
+ * the next kv_batch_* call. */
+kv_status kv_batch_context(kv_batch *b, int i, kv_context **ctx);
+kv_status kv_batch_buffers(kv_batch *b, int i, float **d_vol, float **d_data);
 kv_status kv_batch_get_tiles(kv_batch *b, int i, int *fp_layout, int *bp_slices);
 /* Times the tile shapes per size class (volume voxel count < 16 Mi, < 40 Mi,
  * larger) on this batch and applies the fastest; with a cache file, classes

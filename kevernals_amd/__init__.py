@@ -81,7 +81,10 @@ _SIGS = {
     "kv_trace_rays": (_I, [_P, _I, _I, _P, _I, _P, _P, _P]),
     "kv_forward_project_host": (_I, [_P, _P, _P, _I]),
     "kv_back_project_host": (_I, [_P, _P, _P, _I]),
-    "kv_batch_create": (_I, [_I, _I, _P, ctypes.POINTER(_P)]),
+    "kv_batch_create": (_I, [_I, _I, _P, _I, ctypes.POINTER(_P)]),
+    "kv_batch_context": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "kv_batch_buffers": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "kv_memcpy_d2d": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "kv_batch_destroy": (_I, [_P]),
     "kv_batch_set_volume": (_I, [_P, _I, _P]),
     "kv_batch_set_data": (_I, [_P, _I, _P]),
@@ -187,11 +190,26 @@ class Context:
         h = _P()
         self._check(self.lib.kv_create(device, ctypes.byref(h)))
         self.handle = h
+        self._owned = True
         self.geometry = None
         self.follow_torch_stream = follow_torch_stream
         self._stream = None
         if geometry is not None:
             self.set_geometry(geometry)
+
+    @classmethod
+    def borrow(cls, handle, geometry: Geometry, owner=None) -> "Context":
+        """Wrap a context owned by someone else (a batch's reconstruction):
+        it is never destroyed from here; `owner` is kept alive."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.handle = handle
+        self._owned = False
+        self._owner = owner
+        self.geometry = geometry
+        self.follow_torch_stream = True
+        self._stream = None
+        return self
 
     def _on_torch_stream(self):
         if not self.follow_torch_stream:
@@ -209,7 +227,8 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None):
-            self.lib.kv_destroy(self.handle)
+            if getattr(self, "_owned", True):
+                self.lib.kv_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -219,6 +238,9 @@ class Context:
             pass
 
     # -- plumbing ---------------------------------------------------------
+    def copy(self, dst, src, nbytes: int):
+        """Device-to-device copy on the context's stream."""
+        self._dev(self.lib.kv_memcpy_d2d, _ptr(dst), _ptr(src), nbytes)
     def set_stream(self, stream_handle: Optional[int]):
         """Enqueue on an explicit hipStream_t (0 = null stream) from now on;
         stops following torch's current stream."""
@@ -345,14 +367,28 @@ class Batch:
     replayed from a captured hipGraph.  Host arrays in and out; every call is
     synchronous."""
 
-    def __init__(self, geometries, device: int = 0):
+    def __init__(self, geometries, device: int = 0, mode: int = MODE_R):
         self.lib = load_library()
         self.geometries = list(geometries)
+        self.mode = mode
         keep: list = []
         arr = (_KvGeometry * len(self.geometries))(*[_geometry_struct(g, keep) for g in self.geometries])
         h = _P()
-        self._check(self.lib.kv_batch_create(device, len(self.geometries), ctypes.cast(arr, _P), ctypes.byref(h)))
+        self._check(self.lib.kv_batch_create(device, len(self.geometries), ctypes.cast(arr, _P), mode,
+                                             ctypes.byref(h)))
         self.handle = h
+
+    def context(self, i: int) -> "Context":
+        """Reconstruction i's context (owned by the batch)."""
+        h = _P()
+        self._check(self.lib.kv_batch_context(self.handle, i, ctypes.byref(h)))
+        return Context.borrow(h, self.geometries[i], owner=self)
+
+    def buffers(self, i: int):
+        """Device addresses (volume, data) of reconstruction i."""
+        v, d = _P(), _P()
+        self._check(self.lib.kv_batch_buffers(self.handle, i, ctypes.byref(v), ctypes.byref(d)))
+        return v.value, d.value
 
     def _check(self, status: int):
         if status != KV_OK:

@@ -173,6 +173,7 @@ bool almost_equal_relative(float a, float b)
 }  // namespace
 
 kv_status kv_internal_ensure_tables(kv_context *ctx, int mode) { return ensure_tables(ctx, mode); }
+kv_status kv_internal_ensure_sens(kv_context *ctx) { return ensure_sens(ctx); }
 
 void kv_set_error(const char *fmt, ...)
 {
@@ -342,6 +343,18 @@ kv_status kv_memcpy_d2h(kv_context *ctx, void *h_dst, const void *d_src, size_t 
         return st;
     KV_TRY_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_memcpy_d2d(kv_context *ctx, void *d_dst, const void *d_src, size_t bytes)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    if (!d_dst || !d_src) {
+        kv_set_error("null buffer");
+        return KV_ERR_INVALID;
+    }
+    KV_TRY_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return KV_OK;
 }
 

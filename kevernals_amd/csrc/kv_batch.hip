@@ -22,6 +22,7 @@
 
 struct kv_batch {
     int device = 0;
+    int mode = KV_MODE_R;
     hipStream_t stream = nullptr;
     int n = 0;
     std::vector<kv_context *> ctx;
@@ -102,11 +103,11 @@ hipError_t enqueue_iterations(kv_batch *b, int iters, float relax)
 {
     hipError_t e = kv_launch_zero_u64(b->d_counters, 2 * iters, b->stream);
     for (int k = 0; k < iters && e == hipSuccess; ++k) {
-        e = kv_launch_fp_r_batch(b->d_probs, b->n, b->fp_total, b->d_counters + 2 * k, k, relax, b->fp_grid,
-                                 b->stream);
+        e = kv_launch_fp_batch(b->mode, b->d_probs, b->n, b->fp_total, b->d_counters + 2 * k, k, relax,
+                               b->fp_grid, b->stream);
         if (e == hipSuccess)
-            e = kv_launch_bp_r_batch(b->d_probs, b->n, b->bp_total, b->d_counters + 2 * k + 1, k, b->bp_grid,
-                                     b->stream);
+            e = kv_launch_bp_batch(b->mode, b->d_probs, b->n, b->bp_total, b->d_counters + 2 * k + 1, k,
+                                   b->bp_grid, b->stream);
     }
     return e;
 }
@@ -220,9 +221,9 @@ void release(kv_batch *b)
 
 extern "C" {
 
-kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch **out)
+kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, int mode, kv_batch **out)
 {
-    if (!out || !geoms || n <= 0) {
+    if (!out || !geoms || n <= 0 || (mode != KV_MODE_R && mode != KV_MODE_A)) {
         kv_set_error("bad batch arguments");
         return KV_ERR_INVALID;
     }
@@ -230,6 +231,7 @@ kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch 
     auto *b = new kv_batch();
     b->device = device;
     b->n = n;
+    b->mode = mode;
     kv_status st = KV_OK;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
@@ -254,7 +256,9 @@ kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch 
         if ((st = kv_set_geometry(b->ctx[(size_t)i], &geoms[i])))
             break;
         kv_context *c = b->ctx[(size_t)i];
-        if ((st = kv_internal_ensure_tables(c, KV_MODE_R)))
+        if ((st = kv_internal_ensure_tables(c, mode)))
+            break;
+        if (mode == KV_MODE_A && (st = kv_internal_ensure_sens(c)))
             break;
         const size_t nv = (size_t)kv_nvox(c->g), nr = (size_t)kv_nray(c->g);
         if ((e = hipMalloc((void **)&b->vol[(size_t)i], nv * 4)) != hipSuccess ||
@@ -269,10 +273,11 @@ kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch 
         }
         KvProblem &P = b->probs[(size_t)i];
         P.g = c->g;
-        P.t = c->tab_r;
+        P.t = mode == KV_MODE_A ? c->tab_a : c->tab_r;
         P.vol = b->vol[(size_t)i];
         P.data = b->data[(size_t)i];
         P.ratio = c->d_ratio;
+        P.sens = mode == KV_MODE_A ? c->d_sens : nullptr;
         P.iters = 0;
         P.fp_layout = 1;
         P.bp_zt = 8;
@@ -287,8 +292,8 @@ kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, kv_batch 
         int cus = 0;
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
         cus = cus > 0 ? cus : 1;
-        b->fp_grid = cus * kv_fp_batch_blocks_per_cu();
-        b->bp_grid = cus * kv_bp_batch_blocks_per_cu();
+        b->fp_grid = cus * kv_fp_batch_blocks_per_cu(mode);
+        b->bp_grid = cus * kv_bp_batch_blocks_per_cu(mode);
     }
     if (st != KV_OK) {
         release(b);
@@ -367,6 +372,29 @@ kv_status kv_batch_mlem(kv_batch *b, const int32_t *iters, float relax, int use_
         return KV_OK;
     }
     return run(b, k, relax, use_graph != 0, ms);
+}
+
+kv_status kv_batch_context(kv_batch *b, int i, kv_context **ctx)
+{
+    if (kv_status st = check_index(b, i))
+        return st;
+    if (!ctx) {
+        kv_set_error("null output");
+        return KV_ERR_INVALID;
+    }
+    *ctx = b->ctx[(size_t)i];
+    return KV_OK;
+}
+
+kv_status kv_batch_buffers(kv_batch *b, int i, float **d_vol, float **d_data)
+{
+    if (kv_status st = check_index(b, i))
+        return st;
+    if (d_vol)
+        *d_vol = b->vol[(size_t)i];
+    if (d_data)
+        *d_data = b->data[(size_t)i];
+    return KV_OK;
 }
 
 kv_status kv_batch_set_tiles(kv_batch *b, int i, int fp_layout, int bp_slices)

@@ -579,8 +579,10 @@ __global__ __launch_bounds__(256, 5) void bp_tiled_kernel(KvGeom g, KvTables t, 
 
 // Persistent backprojection + MLEM update over a batch of reconstructions:
 // workgroups claim bricks from a global counter until the list is exhausted.
-__global__ __launch_bounds__(256) void bp_r_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
-                                                         long long total, unsigned long long *counter, int iteration)
+// Mode A divides by each reconstruction's sensitivity A^T 1.
+template <int MODE>
+__global__ __launch_bounds__(256) void bp_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+                                                       long long total, unsigned long long *counter, int iteration)
 {
     __shared__ BpShared<kZT> sm;
     __shared__ long long item_s;
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(256) void bp_r_batch_kernel(const KvProblem *__rest
         const int bz = (int)(local / per_z);
         const int rem = (int)(local - (long long)bz * per_z);
         const int by = rem / P.bp_bx, bx = rem - by * P.bp_bx;
-        bp_brick<KV_MODE_R, KV_EP_MLEM, kZT, false>(P.g, P.t, P.ratio, P.vol, nullptr, bx, by, bz, P.bp_zt, sm);
+        bp_brick<MODE, KV_EP_MLEM, kZT, false>(P.g, P.t, P.ratio, P.vol, P.sens, bx, by, bz, P.bp_zt, sm);
     }
 }
 
@@ -625,10 +627,13 @@ hipError_t launch_tiled(const KvGeom &g, const KvTables &t, const float *proj, f
 
 }  // namespace
 
-hipError_t kv_launch_bp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
-                                int iteration, int grid, hipStream_t s)
+hipError_t kv_launch_bp_batch(int mode, const KvProblem *probs, int nprob, long long total,
+                              unsigned long long *counter, int iteration, int grid, hipStream_t s)
 {
-    bp_r_batch_kernel<<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
+    if (mode == KV_MODE_A)
+        bp_batch_kernel<KV_MODE_A><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
+    else
+        bp_batch_kernel<KV_MODE_R><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
     return hipGetLastError();
 }
 
@@ -638,10 +643,13 @@ hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-int kv_bp_batch_blocks_per_cu()
+int kv_bp_batch_blocks_per_cu(int mode)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bp_r_batch_kernel, 256, 0) != hipSuccess)
+    const hipError_t e =
+        mode == KV_MODE_A ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bp_batch_kernel<KV_MODE_A>, 256, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bp_batch_kernel<KV_MODE_R>, 256, 0);
+    if (e != hipSuccess)
         return 2;
     return n > 0 ? n : 1;
 }

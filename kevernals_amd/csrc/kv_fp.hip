@@ -488,15 +488,11 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
     }
 }
 
+// One ray of the mode-A forward pass with its epilogue.
 template <int EP>
-__global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
-                                                   const float *__restrict__ data)
+__device__ __forceinline__ void fp_a_ray(const KvGeom &g, const float *__restrict__ vol, float *__restrict__ out,
+                                         const float *__restrict__ data, int p, int u, int v)
 {
-    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int p = blockIdx.z;
-    if (u >= g.nu || v >= g.nv)
-        return;
     RayA r;
     ray_setup_a(g, p, u, v, r);
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
@@ -513,6 +509,50 @@ __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__rest
         out[o] = ax;
     else  // MLEM ratio on line-integral data
         out[o] = ax > 0.f ? data[o] / ax : 0.f;
+}
+
+template <int EP>
+__global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
+                                                   const float *__restrict__ data)
+{
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (u >= g.nu || v >= g.nv)
+        return;
+    fp_a_ray<EP>(g, vol, out, data, p, u, v);
+}
+
+// Persistent mode-A forward pass (MLEM ratio) over a batch, as
+// fp_r_batch_kernel.
+__global__ __launch_bounds__(256) void fp_a_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+                                                         long long total, unsigned long long *counter, int iteration)
+{
+    __shared__ long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0)
+            item_s = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long item = item_s;
+        __syncthreads();
+        if (item >= total)
+            break;
+        int pi = 0;
+        while (pi + 1 < nprob && probs[pi + 1].fp_first <= item)
+            ++pi;
+        const KvProblem &P = probs[pi];
+        if (iteration >= P.iters)
+            continue;
+        const long long local = item - P.fp_first;
+        const int per_view = P.fp_tx * P.fp_ty;
+        const int p = (int)(local / per_view);
+        const int rem = (int)(local - (long long)p * per_view);
+        const int ty = rem / P.fp_tx, tx = rem - ty * P.fp_tx;
+        int u, v;
+        fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
+        if (u < P.g.nu && v < P.g.nv)
+            fp_a_ray<KV_EP_MLEM>(P.g, P.vol, P.ratio, P.data, p, u, v);
+    }
 }
 
 // Records the accepted pairs of sampled rays through the same walks.
@@ -581,17 +621,23 @@ hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *
     return hipGetLastError();
 }
 
-hipError_t kv_launch_fp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
-                                int iteration, float relax, int grid, hipStream_t s)
+hipError_t kv_launch_fp_batch(int mode, const KvProblem *probs, int nprob, long long total,
+                              unsigned long long *counter, int iteration, float relax, int grid, hipStream_t s)
 {
-    fp_r_batch_kernel<KV_EP_MLEM><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration, relax);
+    if (mode == KV_MODE_A)
+        fp_a_batch_kernel<<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
+    else
+        fp_r_batch_kernel<KV_EP_MLEM><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration, relax);
     return hipGetLastError();
 }
 
-int kv_fp_batch_blocks_per_cu()
+int kv_fp_batch_blocks_per_cu(int mode)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fp_r_batch_kernel<KV_EP_MLEM>, 256, 0) != hipSuccess)
+    const hipError_t e = mode == KV_MODE_A
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fp_a_batch_kernel, 256, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fp_r_batch_kernel<KV_EP_MLEM>, 256, 0);
+    if (e != hipSuccess)
         return 4;
     return n > 0 ? n : 1;
 }

@@ -59,6 +59,7 @@ struct KvProblem {
     float *vol;
     const float *data;
     float *ratio;
+    const float *sens;         // mode A: sensitivity A^T 1
     int iters;                 // MLEM iterations for this reconstruction
     int fp_layout;             // forward tile layout: 1, 2 or 4 rows per wavefront
     int fp_tx, fp_ty;          // forward tiles per view
@@ -92,6 +93,8 @@ void kv_set_error(const char *fmt, ...);
 
 // Builds the crossing tables of `mode` for the context's geometry (once).
 kv_status kv_internal_ensure_tables(kv_context *ctx, int mode);
+// Computes the mode-A sensitivity A^T 1 (once).
+kv_status kv_internal_ensure_sens(kv_context *ctx);
 
 #define KV_TRY_HIP(expr)                                                                        \
     do {                                                                                        \
@@ -119,15 +122,15 @@ hipError_t kv_launch_bp_r(const KvGeom &g, const KvTables &t, const float *proj,
                           hipStream_t s);
 hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
                           const float *sens, bool unit_input, hipStream_t s);
-// Batched persistent passes (MLEM epilogues, mode R).  `grid` workgroups of
-// 256 threads claim work items from *counter (zero before the launch).
-hipError_t kv_launch_fp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
-                                int iteration, float relax, int grid, hipStream_t s);
-hipError_t kv_launch_bp_r_batch(const KvProblem *probs, int nprob, long long total, unsigned long long *counter,
-                                int iteration, int grid, hipStream_t s);
+// Batched persistent passes (MLEM epilogues, mode R or A).  `grid` workgroups
+// of 256 threads claim work items from *counter (zero before the launch).
+hipError_t kv_launch_fp_batch(int mode, const KvProblem *probs, int nprob, long long total,
+                              unsigned long long *counter, int iteration, float relax, int grid, hipStream_t s);
+hipError_t kv_launch_bp_batch(int mode, const KvProblem *probs, int nprob, long long total,
+                              unsigned long long *counter, int iteration, int grid, hipStream_t s);
 hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
-int kv_fp_batch_blocks_per_cu();
-int kv_bp_batch_blocks_per_cu();
+int kv_fp_batch_blocks_per_cu(int mode);
+int kv_bp_batch_blocks_per_cu(int mode);
 hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes,
                          float *out, hipStream_t s);
 hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s);

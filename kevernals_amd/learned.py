@@ -190,3 +190,85 @@ def time_to_rmse(trace, target: float) -> Optional[float]:
         if r is not None and r <= target:
             return t
     return None
+
+
+@dataclass
+class BatchedRun:
+    trace: List[tuple] = field(default_factory=list)  # (kind, iteration, seconds, [rmse per reconstruction])
+    accepted: List[int] = field(default_factory=list)
+    rejected: List[int] = field(default_factory=list)
+
+
+def batched_gated_mlem(batch, iters: int, params: Optional[np.ndarray] = None, scale: float = DEFAULT_SCALE,
+                       every: int = 2, first: int = 5, patience: Optional[int] = 2, references=None) -> BatchedRun:
+    """MLEM-A on every reconstruction of a mode-A batch (kv_batch: one
+    persistent forward and one persistent backprojection launch per
+    iteration, replayed from a captured hipGraph), with a CNN candidate per
+    reconstruction after iteration `first` and every `every` iterations after
+    it, kept only if that reconstruction's Poisson log-likelihood does not
+    decrease (params None: plain batched MLEM).  Volumes start from the
+    batch's current contents.  With references (one device tensor per
+    reconstruction) the RMSE of every reconstruction is traced after each
+    batch step; wall times exclude those evaluations."""
+    torch = __import__("torch")
+    n = len(batch)
+    ctxs = [batch.context(i) for i in range(n)]
+    bufs = [batch.buffers(i) for i in range(n)]
+    shapes = [g.vol_shape for g in batch.geometries]
+    cnn = Cnn(ctxs[0], params, scale) if params is not None else None
+    run = BatchedRun(accepted=[0] * n, rejected=[0] * n)
+    misses = [0] * n
+    views = [torch.empty(s, device="cuda") for s in shapes] if (references is not None or cnn is not None) else None
+    cand = [torch.empty(s, device="cuda") for s in shapes] if cnn is not None else None
+
+    def fetch(i):
+        ctxs[i].copy(views[i], bufs[i][0], views[i].numel() * 4)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paused = 0.0
+
+    def note(kind, k):
+        nonlocal paused
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0 - paused
+        r = None
+        if references is not None:
+            tp = time.perf_counter()
+            r = []
+            for i in range(n):
+                fetch(i)
+                r.append(rmse(views[i], references[i]))
+            torch.cuda.synchronize()
+            paused += time.perf_counter() - tp
+        run.trace.append((kind, k, t, r))
+
+    k = 0
+    while k < iters:
+        step = min(first if (cnn is not None and k < first) else every, iters - k) if cnn is not None else 1
+        batch.mlem(step, use_graph=True)
+        k += step
+        note("mlem", k)
+        if cnn is None or k < first:
+            continue
+        any_active = False
+        for i in range(n):
+            if patience is not None and misses[i] >= patience:
+                continue
+            any_active = True
+            fetch(i)
+            cnn.forward(views[i], cand[i])
+            data_addr = bufs[i][1]
+            ll_x = ctxs[i].log_likelihood(data_addr, views[i])
+            ll_c = ctxs[i].log_likelihood(data_addr, cand[i])
+            if ll_c >= ll_x:
+                ctxs[i].copy(bufs[i][0], cand[i], cand[i].numel() * 4)
+                run.accepted[i] += 1
+                misses[i] = 0
+            else:
+                run.rejected[i] += 1
+                misses[i] += 1
+        torch.cuda.synchronize()  # the batch stream must see the accepted volumes
+        if any_active:
+            note("cnn", k)
+    return run

@@ -1,5 +1,6 @@
-"""Batched reconstructions (kv_batch_*): many uneven mode-R MLEM problems in
-shared persistent launches, eagerly or replayed from a hipGraph.
+"""Batched reconstructions (kv_batch_*): many uneven MLEM problems (mode R,
+and mode A for the learned path) in shared persistent launches, eagerly or
+replayed from a hipGraph.
 
 Bar: every reconstruction bit-identical to the same number of unbatched
 kv_mlem_iteration calls on its own (which are bit-exact against the
@@ -168,3 +169,47 @@ def test_batch_rejects_bad_arguments(K, setup):
         b.set_tiles(5, 1, 8)
     with pytest.raises(K.KvError):
         b.mlem([-1, 2])
+
+
+def test_batch_mode_a_bit_exact(K, setup):
+    """Mode-A batches (x <- x/s * A^T(data/Ax)) match unbatched MLEM-A."""
+    gs, datas, x0s = setup
+    b = K.Batch(gs, mode=K.MODE_A)
+    for i in range(len(gs)):
+        b.set_volume(i, x0s[i])
+        b.set_data(i, datas[i])
+    iters = [2, 0, 3, 1, 2]
+    b.mlem(iters, use_graph=True)
+    for i, g in enumerate(gs):
+        ctx = K.Context(0, g)
+        x = torch.from_numpy(x0s[i].copy()).cuda()
+        d = torch.from_numpy(datas[i]).cuda()
+        for _ in range(iters[i]):
+            ctx.mlem_iteration(x, d, 1.0, K.MODE_A)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(b.get_volume(i), x.cpu().numpy(), err_msg=f"problem {i}")
+
+
+def test_batched_gate_falls_back_to_plain(K, setup):
+    """A network whose candidates lower the likelihood leaves every
+    reconstruction of a batched gated run identical to plain batched MLEM-A;
+    the per-reconstruction context and buffers address the batch's data."""
+    from kevernals_amd.learned import batched_gated_mlem, init_params
+
+    gs, datas, x0s = setup
+    bad = init_params(3)
+    bad[-1] = 5.0
+    runs = []
+    for params in (None, bad):
+        b = K.Batch(gs, mode=K.MODE_A)
+        for i in range(len(gs)):
+            b.set_volume(i, x0s[i])
+            b.set_data(i, np.maximum(datas[i], 0.0))
+        vol_addr, data_addr = b.buffers(1)
+        assert vol_addr and data_addr
+        run = batched_gated_mlem(b, 6, params=params, every=2, first=2, patience=None)
+        runs.append((run, [b.get_volume(i) for i in range(len(gs))]))
+    (plain, vp), (gated, vg) = runs
+    assert sum(gated.accepted) == 0 and sum(gated.rejected) == 3 * len(gs)
+    for a, c in zip(vp, vg):
+        np.testing.assert_array_equal(a, c)
