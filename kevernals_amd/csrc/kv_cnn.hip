@@ -629,6 +629,10 @@ kv_status kv_cnn_forward(kv_context *ctx, const float *d_params, float scale, co
         kv_set_error("bad CNN arguments");
         return KV_ERR_INVALID;
     }
+    if (d_x == d_y) {
+        kv_set_error("kv_cnn_forward: output must not alias the input (the last layer reads x while writing y)");
+        return KV_ERR_INVALID;
+    }
     const Layout L = layout(nx, ny, nz, false);
     const dim3 grid(L.gx, L.gy, L.gz);
     hipStream_t s = ctx->stream;

@@ -152,6 +152,12 @@ def test_forward_matches_training_pass_and_oracle(K, oracle_mod):
     res = dz4.astype(np.float64) * V / 2.0
     y_from_train = t.astype(np.float64) + res / SCALE
     np.testing.assert_allclose(got, y_from_train, rtol=0, atol=1e-5 * np.abs(got).max())
+    # the output may not alias the input: rejected before any launch
+    xd = torch.from_numpy(x).cuda()
+    with pytest.raises(K.KvError):
+        ctx.cnn_forward(torch.from_numpy(params).cuda(), SCALE, xd, xd, work)
+    with pytest.raises(K.KvError):
+        ctx.cnn_forward(torch.from_numpy(params).cuda(), SCALE, xd, y, work[:16])
 
 
 def test_gradient_vs_fp64_network(K, oracle_mod):
