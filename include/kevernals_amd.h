@@ -217,6 +217,24 @@ kv_status kv_cnn_gradient(kv_context *ctx, const float *d_params, float scale, c
 kv_status kv_cnn_adam(kv_context *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, int step,
                       float lr);
 
+/* ---- trajectory extrapolation of mode-A MLEM ------------------------------
+ * No counterpart in the reference: these serve the accelerated
+ * reconstruction built on the mode-A MLEM update (a jump from iteration k to
+ * an estimate of a later iterate).  d_iterates is a host array of h+1 device
+ * pointers to consecutive MLEM-A iterates x_0..x_h of the context's geometry,
+ * oldest first, 1 <= h <= KV_TRAJ_MAX.  The log-ratios are
+ * d_i = log(x_{i+1}/x_i), i = 0..h-1 (0 where either iterate is not positive).
+ *
+ * kv_traj_gram: gram[i*h + j] = sum_v w(v) d_i(v) d_j(v), w = sens*x_h with
+ * sens = A^T 1 (the context's mode-A sensitivity; 0 where sens or x_h is not
+ * positive).  FP64 accumulation, deterministic; synchronous.
+ * kv_traj_combine: d_out = x_h * exp(sum_i coef[i] d_i) (coef: h host
+ * doubles; x_h kept where it is not positive).  d_out may be one of the
+ * iterates.  Asynchronous on the context's stream. */
+#define KV_TRAJ_MAX 32
+kv_status kv_traj_gram(kv_context *ctx, const float *const *d_iterates, int h, double *gram);
+kv_status kv_traj_combine(kv_context *ctx, const float *const *d_iterates, int h, const double *coef, float *d_out);
+
 /* ---- host-buffer variants (synchronous; used by the C++ Projector) ------- */
 kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode);
 kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode);

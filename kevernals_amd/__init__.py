@@ -99,6 +99,8 @@ _SIGS = {
     "kv_cnn_gradient": (_I, [_P, _P, _F, _P, _P, _I, _I, _I, _P, ctypes.c_size_t, _P,
                              ctypes.POINTER(ctypes.c_double)]),
     "kv_cnn_adam": (_I, [_P, _P, _P, _P, _P, _I, _F]),
+    "kv_traj_gram": (_I, [_P, _P, _I, _P]),
+    "kv_traj_combine": (_I, [_P, _P, _I, _P, _P]),
 }
 
 
@@ -147,6 +149,7 @@ def _nbytes(x) -> int:
 
 
 CNN_NPARAMS = 14737
+TRAJ_MAX = 32
 
 
 def cnn_workspace_size(shape, training: bool) -> int:
@@ -336,6 +339,25 @@ class Context:
 
     def cnn_adam(self, params, grad, m, v, step: int, lr: float = 1e-3):
         self._dev(self.lib.kv_cnn_adam, _ptr(params), _ptr(grad), _ptr(m), _ptr(v), step, lr)
+
+    def traj_gram(self, iterates) -> np.ndarray:
+        """Gram matrix (h x h, FP64) of the log-ratios of h+1 consecutive
+        mode-A MLEM iterates (device volumes, oldest first) under the weight
+        sens * newest iterate."""
+        h = len(iterates) - 1
+        ptrs = (ctypes.c_void_p * (h + 1))(*[_ptr(x) for x in iterates])
+        out = np.zeros((max(h, 0), max(h, 0)), dtype=np.float64)
+        self._dev(self.lib.kv_traj_gram, ctypes.cast(ptrs, ctypes.c_void_p), h, _host_ptr(out))
+        return out
+
+    def traj_combine(self, iterates, coef, out):
+        """out = newest iterate * exp(sum_i coef[i] * log(x_{i+1}/x_i))."""
+        h = len(iterates) - 1
+        ptrs = (ctypes.c_void_p * (h + 1))(*[_ptr(x) for x in iterates])
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        if c.size != h:
+            raise ValueError(f"{h} log-ratios need {h} coefficients, got {c.size}")
+        self._dev(self.lib.kv_traj_combine, ctypes.cast(ptrs, ctypes.c_void_p), h, _host_ptr(c), _ptr(out))
 
     def forward_project_host(self, vol: np.ndarray, mode: int = MODE_R) -> np.ndarray:
         g = self.geometry

@@ -624,6 +624,96 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
     return KV_OK;
 }
 
+namespace {
+kv_status traj_args(kv_context *ctx, const float *const *d_iterates, int h, KvTraj &tr)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_iterates || h < 1 || h > KV_TRAJ_MAX) {
+        kv_set_error("trajectory needs 1..%d log-ratios (h = %d)", KV_TRAJ_MAX, h);
+        return KV_ERR_INVALID;
+    }
+    for (int i = 0; i <= h; ++i) {
+        if (!d_iterates[i]) {
+            kv_set_error("null iterate %d", i);
+            return KV_ERR_INVALID;
+        }
+        tr.x[i] = d_iterates[i];
+    }
+    for (int i = h + 1; i <= KV_TRAJ_MAX; ++i)
+        tr.x[i] = nullptr;
+    return KV_OK;
+}
+}  // namespace
+
+kv_status kv_traj_gram(kv_context *ctx, const float *const *d_iterates, int h, double *gram)
+{
+    KvTraj tr;
+    if (kv_status st = traj_args(ctx, d_iterates, h, tr))
+        return st;
+    if (!gram) {
+        kv_set_error("null gram");
+        return KV_ERR_INVALID;
+    }
+    if (kv_status st = ensure_sens(ctx))
+        return st;
+    const int64_t n = kv_nvox(ctx->g);
+    const int nb = kv_traj_gram_blocks(n), np = kv_traj_pairs(h);
+    const int need = nb * np * 16;
+    if (ctx->red_cap < need) {
+        free_dev(ctx->d_red);
+        if (kv_status st = dev_alloc((void **)&ctx->d_red, (size_t)need * sizeof(double)))
+            return st;
+        ctx->red_cap = need;
+    }
+    KV_TRY_HIP(kv_launch_traj_gram(tr, h, n, ctx->d_sens, ctx->d_red, ctx->stream));
+    std::vector<double> part((size_t)need);
+    KV_TRY_HIP(hipMemcpyAsync(part.data(), ctx->d_red, (size_t)need * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    // workgroup partials in workgroup order; block pair p = (bi, bj), bi <= bj,
+    // row-major over the upper triangle of 4x4 blocks
+    const int nbk = (h + 3) / 4;
+    std::vector<double> blk((size_t)np * 16, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int o = 0; o < np * 16; ++o)
+            blk[o] += part[(size_t)b * np * 16 + o];
+    int p = 0;
+    for (int bi = 0; bi < nbk; ++bi)
+        for (int bj = bi; bj < nbk; ++bj, ++p)
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) {
+                    const int i = 4 * bi + r, j = 4 * bj + c;
+                    if (i >= h || j >= h)
+                        continue;
+                    const double v = blk[(size_t)p * 16 + r * 4 + c];
+                    gram[i * h + j] = v;
+                    gram[j * h + i] = v;
+                }
+    return KV_OK;
+}
+
+kv_status kv_traj_combine(kv_context *ctx, const float *const *d_iterates, int h, const double *coef, float *d_out)
+{
+    KvTraj tr;
+    if (kv_status st = traj_args(ctx, d_iterates, h, tr))
+        return st;
+    if (!coef || !d_out) {
+        kv_set_error("null coefficients or output");
+        return KV_ERR_INVALID;
+    }
+    KvTrajCoef cf;
+    for (int i = 0; i < KV_TRAJ_MAX; ++i) {
+        cf.c[i] = i < h ? coef[i] : 0.0;
+        if (!std::isfinite(cf.c[i])) {
+            kv_set_error("coefficient %d is not finite", i);
+            return KV_ERR_INVALID;
+        }
+    }
+    KV_TRY_HIP(kv_launch_traj_combine(tr, cf, h, kv_nvox(ctx->g), d_out, ctx->stream));
+    return KV_OK;
+}
+
 kv_status kv_trace_rays(kv_context *ctx, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx, float *w,
                         int32_t *counts)
 {

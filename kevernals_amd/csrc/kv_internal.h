@@ -136,3 +136,19 @@ hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_
 hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s);
 hipError_t kv_launch_loglik(const KvGeom &g, const float *data, const float *ax, double *partials, int nblocks,
                             hipStream_t s);
+
+// ---- trajectory extrapolation (kv_traj.hip) ------------------------------
+struct KvTraj {
+    const float *x[KV_TRAJ_MAX + 1];  // iterates, oldest first
+};
+struct KvTrajCoef {
+    double c[KV_TRAJ_MAX];
+};
+// Workgroups of the Gram pass (fixed for a volume size) and the number of
+// 4x4 blocks of the upper triangle it writes per workgroup (16 doubles each).
+int kv_traj_gram_blocks(int64_t n);
+int kv_traj_pairs(int h);
+hipError_t kv_launch_traj_gram(const KvTraj &tr, int h, int64_t n, const float *sens, double *partials,
+                               hipStream_t s);
+hipError_t kv_launch_traj_combine(const KvTraj &tr, const KvTrajCoef &cf, int h, int64_t n, float *out,
+                                  hipStream_t s);
