@@ -1,5 +1,7 @@
 """Learned-accelerated MLEM: training pairs, CNN training and the
-likelihood-gated interleaving of CNN steps with MLEM iterations.
+likelihood-gated interleaving of CNN steps with MLEM iterations; the
+trajectory-jump schedule (kevernals_amd.trajectory) with the CNN applied to
+its result.
 
 Everything numerical runs on the GPU through the C ABI (kv_mlem_iteration
 in mode A, kv_log_likelihood, kv_cnn_*); this module only sequences calls.
@@ -271,4 +273,209 @@ def batched_gated_mlem(batch, iters: int, params: Optional[np.ndarray] = None, s
         torch.cuda.synchronize()  # the batch stream must see the accepted volumes
         if any_active:
             note("cnn", k)
+    return run
+
+
+def _gate(ctx, data, x, candidates, ll_x=None):
+    """First candidate whose Poisson log-likelihood is not below x's: (index,
+    its log-likelihood, x's), or (None, None, x's)."""
+    if ll_x is None:
+        ll_x = ctx.log_likelihood(data, x)
+    for i, c in enumerate(candidates):
+        ll_c = ctx.log_likelihood(data, c)
+        if ll_c >= ll_x:
+            return i, ll_c, ll_x
+    return None, None, ll_x
+
+
+def accelerated_mlem(ctx: Context, data, schedule=None, cnn: Optional[Cnn] = None, iters: Optional[int] = None,
+                     reference=None, target_rmse: Optional[float] = None, x0=None) -> GatedRun:
+    """MLEM-A with the two-stretch trajectory jump (trajectory.Schedule) and,
+    with a network, a CNN step on the final jump's result.  Every step
+    outside plain MLEM is gated: the first jump, and the final candidates
+    (CNN of the final jump, then the final jump alone) replace the estimate
+    only if its Poisson log-likelihood does not decrease; a rejected first
+    jump ends the schedule (plain MLEM from there on).  `iters` MLEM
+    iterations run in total (default: the schedule's cost, so the run ends
+    with the final jump).  Trace kinds: mlem, jump+/jump- (first jump),
+    final+/final- (the final jump), cnn+ (the CNN step kept with it).  With
+    a reference the RMSE after every step is traced (excluded from the
+    times); with target_rmse the run stops once it is reached."""
+    from .trajectory import Jumper, Schedule
+
+    torch = __import__("torch")
+    sch = schedule or Schedule()
+    iters = sch.cost if iters is None else iters
+    g = ctx.geometry
+    x = torch.ones(g.vol_shape, device="cuda") if x0 is None else x0.clone()
+    jm = Jumper(ctx, g.vol_shape, sch)
+    cand = torch.empty_like(x)
+    cand2 = torch.empty_like(x) if cnn is not None else None
+    run = GatedRun(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paused = 0.0
+
+    def note(kind, k, ll=None):
+        nonlocal paused
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0 - paused
+        r = None
+        if reference is not None:
+            tp = time.perf_counter()
+            r = rmse(x, reference)
+            paused += time.perf_counter() - tp
+        run.trace.append((kind, k, t, r, ll))
+        return r is not None and target_rmse is not None and r <= target_rmse
+
+    active = True
+    if sch.start == 0:
+        jm.record(x)
+    k = 0
+    while k < iters:
+        ctx.mlem_iteration(x, data, 1.0, MODE_A)
+        k += 1
+        if active and (sch.start <= k <= sch.first or jm.count > jm.na):
+            jm.record(x)
+        if note("mlem", k):
+            break
+        if active and k == sch.first:
+            jm.first_jump(cand)
+            i, ll_c, _ = _gate(ctx, data, x, [cand])
+            if i is None:
+                active = False
+                run.rejected += 1
+                if note("jump-", k):
+                    break
+                continue
+            x.copy_(cand)
+            jm.record(x)
+            run.accepted += 1
+            if note("jump+", k, ll_c):
+                break
+        elif active and k == sch.cost:
+            active = False
+            jm.final_jump(cand)
+            cands = [cand]
+            if cnn is not None:
+                cnn.forward(cand, cand2)
+                cands = [cand2, cand]
+            i, ll_c, _ = _gate(ctx, data, x, cands)
+            if i is None:
+                run.rejected += 1
+                if note("final-", k):
+                    break
+                continue
+            x.copy_(cands[i])
+            run.accepted += 1
+            if cnn is not None and i == 0:
+                run.accepted += 1
+                done = note("cnn+", k, ll_c)
+            else:
+                if cnn is not None:
+                    run.rejected += 1
+                done = note("final+", k, ll_c)
+            if done:
+                break
+    run.x = x
+    return run
+
+
+def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] = None,
+                             scale: float = DEFAULT_SCALE, iters: Optional[int] = None,
+                             references=None) -> BatchedRun:
+    """The two-stretch jump schedule on every reconstruction of a mode-A batch:
+    the MLEM iterations of all reconstructions run as one persistent forward
+    and one persistent backprojection launch per iteration (captured
+    hipGraph), each reconstruction records its own stretches and makes its
+    own gated jumps (and CNN step, with params) through its batch context.
+    Volumes start from the batch's current contents."""
+    from .trajectory import Jumper, Schedule
+
+    torch = __import__("torch")
+    sch = schedule or Schedule()
+    iters = sch.cost if iters is None else iters
+    n = len(batch)
+    ctxs = [batch.context(i) for i in range(n)]
+    bufs = [batch.buffers(i) for i in range(n)]
+    shapes = [g.vol_shape for g in batch.geometries]
+    cnn = Cnn(ctxs[0], params, scale) if params is not None else None
+    views = [torch.empty(s, device="cuda") for s in shapes]
+    cand = [torch.empty(s, device="cuda") for s in shapes]
+    cand2 = [torch.empty(s, device="cuda") for s in shapes] if cnn is not None else None
+    jms = [Jumper(ctxs[i], shapes[i], sch) for i in range(n)]
+    active = [True] * n
+    run = BatchedRun(accepted=[0] * n, rejected=[0] * n)
+
+    def fetch(i):
+        ctxs[i].copy(views[i], bufs[i][0], views[i].numel() * 4)
+
+    def store(i, src):
+        ctxs[i].copy(bufs[i][0], src, src.numel() * 4)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paused = 0.0
+
+    def note(kind, k):
+        nonlocal paused
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0 - paused
+        r = None
+        if references is not None:
+            tp = time.perf_counter()
+            r = []
+            for i in range(n):
+                fetch(i)
+                r.append(rmse(views[i], references[i]))
+            torch.cuda.synchronize()
+            paused += time.perf_counter() - tp
+        run.trace.append((kind, k, t, r))
+
+    if sch.start == 0:
+        for i in range(n):
+            fetch(i)
+            jms[i].record(views[i])
+    k = 0
+    while k < iters:
+        batch.mlem(1, use_graph=True)
+        k += 1
+        for i in range(n):
+            if active[i] and (sch.start <= k <= sch.first or jms[i].count > jms[i].na):
+                fetch(i)
+                jms[i].record(views[i])
+        note("mlem", k)
+        if k == sch.first or k == sch.cost:
+            for i in range(n):
+                if not active[i]:
+                    continue
+                fetch(i)
+                data_addr = bufs[i][1]
+                if k == sch.first:
+                    jms[i].first_jump(cand[i])
+                    cands = [cand[i]]
+                else:
+                    active[i] = False
+                    jms[i].final_jump(cand[i])
+                    cands = [cand[i]]
+                    if cnn is not None:
+                        cnn.forward(cand[i], cand2[i])
+                        cands = [cand2[i], cand[i]]
+                j, _, _ = _gate(ctxs[i], data_addr, views[i], cands)
+                if j is None:
+                    active[i] = False
+                    run.rejected[i] += 1
+                    continue
+                store(i, cands[j])
+                run.accepted[i] += 1
+                if k == sch.first:
+                    jms[i].record(cands[j])
+                elif cnn is not None:
+                    # the CNN candidate kept (j == 0) or passed over for the jump alone
+                    if j == 0:
+                        run.accepted[i] += 1
+                    else:
+                        run.rejected[i] += 1
+            torch.cuda.synchronize()  # the batch stream must see the new volumes
+            note("jump" if k == sch.first else "final", k)
     return run

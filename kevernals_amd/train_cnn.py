@@ -3,7 +3,10 @@ itself: 5-iteration MLEM (input) and 100-iteration MLEM (target) of seeded
 phantoms, 256x256x32 voxels at 0.25x0.25x2 mm, 512^2 detector at 0.3 mm,
 15 views on +-15 deg, transmission data at I0 = 1e4.  L2 loss, Adam.
 --stages 5,10,20,40 adds the same run's later estimates as inputs (each
-paired with its 100-iteration estimate).
+paired with its 100-iteration estimate).  --input krylov pairs the
+trajectory-extrapolated estimate made at iteration --at (from the last
+--history log-ratios, kevernals_amd.trajectory) with the 100-iteration
+estimate instead: the network then corrects what the extrapolation misses.
 
     python -m kevernals_amd.train_cnn --pairs 2000 --epochs 4 --out weights.npz
 
@@ -33,6 +36,41 @@ def _phantom(args):
 
 def log(msg):
     print(msg, flush=True)
+
+
+def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref, gen):
+    """(input, target) pairs: the trajectory extrapolation made at iteration
+    `at` from the last `history` log-ratios, jumping to iteration k_ref,
+    paired with the k_ref-iteration estimate of the same run."""
+    import torch
+
+    from kevernals_amd.learned import transmission_data
+    from kevernals_amd.trajectory import Trajectory
+
+    g = ctx.geometry
+    tr = Trajectory(ctx, g.vol_shape, history)
+    xs, ts = [], []
+    t0 = time.time()
+    for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
+        mu = torch.from_numpy(mu_h).cuda()
+        data = transmission_data(ctx, mu, i0, gen)
+        x = torch.ones(g.vol_shape, device="cuda")
+        tr.reset()
+        if at - history <= 0:
+            tr.push(x)
+        for k in range(1, k_ref + 1):
+            ctx.mlem_iteration(x, data, 1.0, 1)
+            if at - history <= k <= at:
+                tr.push(x)
+            if k == at:
+                y = torch.empty_like(x)
+                tr.extrapolate(k_ref - at, y)
+                xs.append(y)
+        ts.append(x)
+        if (n + 1) % 50 == 0 or n + 1 == len(seeds):
+            torch.cuda.synchronize()
+            log(f"[train] pairs {n + 1}/{len(seeds)} in {time.time() - t0:.1f} s")
+    return xs, ts
 
 
 def make_pairs(ctx, seeds, pool, i0, stages, k_ref, gen):
@@ -79,6 +117,9 @@ def main(argv=None):
     ap.add_argument("--stages", default=None,
                     help="comma-separated input iteration counts (default: --k-in only)")
     ap.add_argument("--k-ref", type=int, default=100)
+    ap.add_argument("--input", choices=["mlem", "krylov"], default="mlem")
+    ap.add_argument("--at", type=int, default=20, help="--input krylov: iteration of the jump")
+    ap.add_argument("--history", type=int, default=18, help="--input krylov: log-ratios used")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="cnn_weights.npz")
@@ -100,10 +141,15 @@ def main(argv=None):
     t0 = time.time()
     stages = sorted({int(v) for v in args.stages.split(",")}) if args.stages else [args.k_in]
     assert 0 < stages[0] and stages[-1] < args.k_ref
-    xs, ts = make_pairs(ctx, list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs)), pool, args.i0, stages,
-                        args.k_ref, gen)
-    vx, vt = make_pairs(ctx, list(range(VAL_SEED0, VAL_SEED0 + args.val)), pool, args.i0, [args.k_in], args.k_ref,
-                        gen)
+    train_seeds = list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs))
+    val_seeds = list(range(VAL_SEED0, VAL_SEED0 + args.val))
+    if args.input == "krylov":
+        xs, ts = make_krylov_pairs(ctx, train_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
+        vx, vt = make_krylov_pairs(ctx, val_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
+        stages = [args.at]
+    else:
+        xs, ts = make_pairs(ctx, train_seeds, pool, args.i0, stages, args.k_ref, gen)
+        vx, vt = make_pairs(ctx, val_seeds, pool, args.i0, [args.k_in], args.k_ref, gen)
     pool.close()
     pool.join()
     t_data = time.time() - t0
@@ -119,7 +165,7 @@ def main(argv=None):
         return float(np.mean([float(((a - b).double() * s).pow(2).mean()) for a, b in zip(vx, vt)]))
 
     base = identity_loss()
-    log(f"[train] validation loss of the 5-iteration input itself: {base:.5e}; untrained CNN {val_loss():.5e}")
+    log(f"[train] validation loss of the input itself: {base:.5e}; untrained CNN {val_loss():.5e}")
     rng = np.random.Generator(np.random.PCG64(args.seed))
     history = []
     t1 = time.time()
@@ -134,6 +180,7 @@ def main(argv=None):
     steps_per_s = cnn.step / (time.time() - t1)
     cnn.save(args.out, history=np.array(history), identity_val=base, pairs=len(xs), phantoms=args.pairs,
              k_in=args.k_in, stages=np.array(stages), k_ref=args.k_ref, i0=args.i0, data_seconds=t_data,
+             input_kind=args.input, at=args.at, krylov_history=args.history,
              steps_per_s=steps_per_s)
     log(f"[train] saved {args.out}: {cnn.step} steps, {steps_per_s:.1f} steps/s")
     return 0

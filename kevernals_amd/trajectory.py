@@ -29,24 +29,40 @@ import numpy as np
 from . import TRAJ_MAX, Context
 
 
-def extrapolation_coefficients(gram: np.ndarray, horizon: int, rel_tol: float = 1e-11,
-                               ritz_max: float = 1.0 - 1e-5):
-    """Coefficients c (length h) with log x_{h+horizon} ~ log x_h + sum_i c_i d_i,
-    and the Ritz values used.  Directions of the history whose Gram
-    eigenvalue is below rel_tol times the largest are dropped (they are
-    rounding noise of the FP32 iterates); Ritz values are clamped to
-    [0, ritz_max] (EM's Jacobian has its spectrum in [0, 1))."""
+def ritz_coefficients(gram: np.ndarray, pairs, target: int, horizon: int, rel_tol: float = 1e-11,
+                      ritz_max: float = 1.0 - 1e-5):
+    """Coefficients c over all h log-ratios with
+        sum_i c_i d_i ~ p(J) J d_target,   p(t) = sum_{i=1..horizon} t^i,
+    from a Gram matrix of h log-ratios and the list of pairs (i, j) with
+    d_j = J d_i (consecutive log-ratios of one stretch of MLEM iterations;
+    the pairs may come from several stretches).  d_target must be one of the
+    pairs' first members.  Returns (c, Ritz values).
+
+    Rayleigh-Ritz on the span of the pairs' first members B: Gram matrices
+    B^T W B and B^T W JB, a W-orthonormal basis B Q from the eigenvectors of
+    B^T W B whose eigenvalues exceed rel_tol times the largest (the rest is
+    rounding noise of FP32 iterates), T = Q^T (B^T W JB) Q symmetrised, Ritz
+    values clamped to [0, ritz_max] (EM's Jacobian has its spectrum in
+    [0, 1))."""
     G = np.asarray(gram, dtype=np.float64)
     h = G.shape[0]
     if G.shape != (h, h) or h < 2:
         raise ValueError(f"need an h x h Gram matrix with h >= 2, got {G.shape}")
     if horizon < 0:
         raise ValueError("horizon must be >= 0")
-    n = h - 1
-    Gb, H = G[:n, :n], G[:n, 1:]
+    pairs = [(int(i), int(j)) for i, j in pairs]
+    if not pairs or any(not (0 <= i < h and 0 <= j < h) for i, j in pairs):
+        raise ValueError("pairs must index the Gram matrix")
+    bi = [i for i, _ in pairs]
+    bj = [j for _, j in pairs]
+    if target not in bi:
+        raise ValueError("the target must be the first member of a pair")
+    Gb = G[np.ix_(bi, bi)]
+    H = G[np.ix_(bi, bj)]
     ev, U = np.linalg.eigh(Gb)
+    coef = np.zeros(h)
     if not ev.size or ev[-1] <= 0.0 or horizon == 0:
-        return np.zeros(h), np.zeros(0)
+        return coef, np.zeros(0)
     keep = ev > ev[-1] * rel_tol
     Q = U[:, keep] / np.sqrt(ev[keep])
     T = Q.T @ H @ Q
@@ -54,9 +70,41 @@ def extrapolation_coefficients(gram: np.ndarray, horizon: int, rel_tol: float = 
     th, Y = np.linalg.eigh(T)
     th = np.clip(th, 0.0, ritz_max)
     p = th * (1.0 - th ** horizon) / (1.0 - th)
-    c = Q.T @ Gb[:, n - 1]
-    coef = Q @ (Y @ (p * th * (Y.T @ c)))
-    return np.append(coef, 0.0), th
+    c = Q.T @ Gb[:, bi.index(target)]
+    cb = Q @ (Y @ (p * th * (Y.T @ c)))
+    for k, i in enumerate(bi):
+        coef[i] += cb[k]
+    return coef, th
+
+
+def extrapolation_coefficients(gram: np.ndarray, horizon: int, rel_tol: float = 1e-11,
+                               ritz_max: float = 1.0 - 1e-5):
+    """One stretch of h+1 consecutive iterates: coefficients c (length h) with
+    log x_{h+horizon} ~ log x_h + sum_i c_i d_i (d_{h-1} = J d_{h-2}, so the
+    remaining path is p(J) J d_{h-2}), and the Ritz values used."""
+    G = np.asarray(gram, dtype=np.float64)
+    h = G.shape[0]
+    if G.shape != (h, h) or h < 2:
+        raise ValueError(f"need an h x h Gram matrix with h >= 2, got {G.shape}")
+    return ritz_coefficients(G, [(i, i + 1) for i in range(h - 1)], h - 2, horizon, rel_tol, ritz_max)
+
+
+def two_stretch_coefficients(gram: np.ndarray, na: int, horizon: int, rel_tol: float = 1e-11):
+    """Iterates x_0..x_{na-1} of stretch A, then a jump, then stretch B
+    (the jump's result and the MLEM iterates after it), as one list of h+1
+    iterates: coefficients c over its h log-ratios such that
+    x_newest * exp(sum_i c_i d_i) estimates the iterate `horizon` iterations
+    after x_{na-1}, and the Ritz values used.  Log-ratios 0..na-2 belong to
+    stretch A, na-1 is the jump (no J-pair), na..h-1 to stretch B."""
+    G = np.asarray(gram, dtype=np.float64)
+    h = G.shape[0]
+    if na < 4 or h - na < 2:
+        raise ValueError("need at least 4 iterates in stretch A and 3 in stretch B")
+    pairs = [(i, i + 1) for i in range(na - 2)] + [(i, i + 1) for i in range(na, h - 1)]
+    coef, ritz = ritz_coefficients(G, pairs, na - 3, horizon, rel_tol)
+    # combination base: log x_{na-1} = log x_newest - (jump + stretch B log-ratios)
+    coef[na - 1:] -= 1.0
+    return coef, ritz
 
 
 @dataclass
@@ -105,3 +153,75 @@ class Trajectory:
         coef, ritz = extrapolation_coefficients(G, horizon, rel_tol)
         self.ctx.traj_combine(xs, coef, out)
         return Extrapolation(coef, ritz, horizon, len(xs) - 1)
+
+
+@dataclass
+class Schedule:
+    """Two-stretch jump schedule.  MLEM-A runs `first` iterations, recording
+    the iterates from `start` on (stretch A); the first jump takes the
+    estimate to iteration `mid`; `second` MLEM iterations run from there
+    (stretch B: their log-ratios are filtered by about mid powers of J, so
+    they span the slow modes that stretch A cannot resolve); the final jump
+    starts again from iteration `first`, with the Ritz basis of both
+    stretches, and lands on an estimate of iteration `horizon`.  Cost: first
+    + second MLEM iterations, two Gram passes and two combinations."""
+    start: int = 2
+    first: int = 10
+    mid: int = 40
+    second: int = 8
+    horizon: int = 100
+
+    def __post_init__(self):
+        na = self.first - self.start + 1
+        if self.start < 0 or na < 4:
+            raise ValueError("stretch A needs at least 4 iterates (first - start >= 3)")
+        if self.second < 2:
+            raise ValueError("stretch B needs at least 2 iterations")
+        if na + self.second > TRAJ_MAX + 1:
+            raise ValueError(f"the two stretches hold at most {TRAJ_MAX + 1} iterates")
+        if not self.first < self.mid < self.horizon:
+            raise ValueError("need first < mid < horizon")
+
+    @property
+    def cost(self) -> int:
+        return self.first + self.second
+
+
+class Jumper:
+    """The recorded stretches of one reconstruction and its two jumps
+    (device copies; one context per reconstruction)."""
+
+    def __init__(self, ctx: Context, shape, schedule: Schedule):
+        import torch
+
+        self.ctx = ctx
+        self.s = schedule
+        self.na = schedule.first - schedule.start + 1
+        self.slots = [torch.empty(shape, device="cuda") for _ in range(self.na + schedule.second + 1)]
+        self.count = 0
+
+    def record(self, x):
+        if self.count >= len(self.slots):
+            raise RuntimeError("both stretches are recorded already")
+        self.ctx.copy(self.slots[self.count], x, x.numel() * 4)
+        self.count += 1
+
+    def first_jump(self, out) -> Extrapolation:
+        """out = estimate of iteration `mid` from stretch A (complete)."""
+        if self.count != self.na:
+            raise RuntimeError("stretch A is not complete")
+        xs = self.slots[:self.na]
+        coef, ritz = extrapolation_coefficients(self.ctx.traj_gram(xs), self.s.mid - self.s.first)
+        self.ctx.traj_combine(xs, coef, out)
+        return Extrapolation(coef, ritz, self.s.mid - self.s.first, self.na - 1)
+
+    def final_jump(self, out) -> Extrapolation:
+        """out = estimate of iteration `horizon`, from iteration `first`, with
+        the Ritz basis of both stretches (complete: the first jump's result
+        and the `second` iterations after it recorded)."""
+        if self.count != len(self.slots):
+            raise RuntimeError("stretch B is not complete")
+        horizon = self.s.horizon - self.s.first
+        coef, ritz = two_stretch_coefficients(self.ctx.traj_gram(self.slots), self.na, horizon)
+        self.ctx.traj_combine(self.slots, coef, out)
+        return Extrapolation(coef, ritz, horizon, len(self.slots) - 1)

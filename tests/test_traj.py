@@ -90,3 +90,35 @@ def test_truncation_drops_rounding_noise():
     y = traj.combine(its, coef)
     np.testing.assert_allclose(np.log(y), np.log(x(10 + 30)), rtol=0, atol=2e-5)
     assert ritz.size < 7
+
+
+def evolve(L, lam, z0, j):
+    """Iterate j of the diagonal log-linear recurrence started from z0."""
+    return np.exp(L + lam ** j * (np.log(z0) - L))
+
+
+@pytest.mark.parametrize("lams", [[0.5, 0.9, 0.97], [0.2, 0.6, 0.85, 0.95, 0.99]])
+def test_two_stretch_jump_exact_on_linear_recurrence(lams):
+    """Stretch A of a log-linear recurrence, an arbitrary jump, stretch B of
+    the same recurrence from the jump's result: the final jump from the end
+    of stretch A lands exactly on the recurrence's later iterate (the
+    combined basis spans every mode).  A wrong pair (the jump treated as a
+    J-step) or a missing rebase to stretch A's end fails this."""
+    from kevernals_amd.trajectory import two_stretch_coefficients
+
+    r = rng(5)
+    n = 3000
+    L = np.log(0.02 + 0.05 * r.random(n))
+    lam = np.asarray(lams)[r.integers(0, len(lams), n)]
+    x0 = np.exp(L + 0.5 * r.standard_normal(n))
+    sens = 0.5 + r.random(n)
+    A = [evolve(L, lam, x0, j) for j in range(2, 11)]  # iterates 2..10
+    z = np.exp(L + 0.3 * r.standard_normal(n))  # the jump's result: any state
+    B = [evolve(L, lam, z, j) for j in range(0, 9)]
+    its = A + B
+    for horizon in (5, 40, 90):
+        coef, ritz = two_stretch_coefficients(traj.gram(its, sens), len(A), horizon)
+        y = traj.combine(its, coef)
+        np.testing.assert_allclose(np.log(y), np.log(evolve(L, lam, x0, 10 + horizon)), rtol=0, atol=2e-6)
+    with pytest.raises(ValueError):  # two iterates in stretch B: no J-pair there
+        two_stretch_coefficients(traj.gram(its[:11], sens), 9, 10)
