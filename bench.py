@@ -235,16 +235,53 @@ def bench_cnn(args, torch, K, G):
     print(json.dumps(out), flush=True)
 
 
+def _times(trace, thr, rmse_of):
+    """(seconds, MLEM iterations run) at the first trace entry with RMSE <= thr."""
+    for e in trace:
+        r = rmse_of(e)
+        if r is not None and r <= thr:
+            return e[2], e[1]
+    return None, None
+
+
+def _thresholds(runs, rmse_of, label):
+    out = {}
+    for thr in (0.10, 0.05, 0.02, 0.01):
+        row = {}
+        for name, run in runs.items():
+            t, k = _times(run.trace, thr, rmse_of)
+            row[f"{name}_s"] = None if t is None else round(t, 4)
+            row[f"{name}_iters"] = k
+        tp = row["plain_s"]
+        for name in runs:
+            if name != "plain":
+                tl = row[f"{name}_s"]
+                row[f"{name}_speedup"] = round(tp / tl, 3) if (tp is not None and tl) else None
+        out[f"{thr:g}"] = row
+        log(f"[bench] {label} RMSE <= {thr:g}: " + ", ".join(
+            f"{n} {row[n + '_s']} s ({row[n + '_iters']} it)" for n in runs))
+    return out
+
+
+def _load_cnn_params(name):
+    with np.load(os.path.join(ROOT, "kevernals_amd", "weights", name), allow_pickle=False) as f:
+        return f["params"], float(f["scale"])
+
+
 def bench_learned_batched(args, torch, K, G):
     """Learned plus batched at full size: `--recons` reconstructions of
     1024x1024x64 / 25 views (phantom seeds 5000+i with 200
     microcalcifications, I0 = 5e3) in one mode-A batch (persistent launches,
-    hipGraph replay), plain batched MLEM-A vs the same batch with the
-    likelihood-gated CNN (after iteration 5, then every 2 iterations).  Time
-    to every reconstruction's RMSE <= threshold against its own 100-iteration
-    MLEM-A result; times exclude the RMSE evaluations."""
-    from kevernals_amd.learned import batched_gated_mlem, transmission_data
+    hipGraph replay).  Plain batched MLEM-A against the accelerated batched
+    reconstruction: the two-stretch trajectory-jump schedule (18 batched
+    MLEM iterations; each reconstruction's jumps gated on its Poisson
+    log-likelihood) with the likelihood-gated CNN step on its result
+    (weights cnn_schedule_to_100.npz); the schedule without the network is
+    timed too.  Time to every reconstruction's RMSE <= threshold against its
+    own 100-iteration MLEM-A result; times exclude the RMSE evaluations."""
+    from kevernals_amd.learned import batched_accelerated_mlem, batched_gated_mlem, transmission_data
     from kevernals_amd.phantom import phantom
+    from kevernals_amd.trajectory import Schedule
 
     g = G.full_plain()
     n = args.recons
@@ -269,6 +306,7 @@ def bench_learned_batched(args, torch, K, G):
     log(f"[bench] {n} reconstructions ready in {time.time() - t:.1f} s")
     t = time.perf_counter()
     b.mlem(100, use_graph=True)
+    torch.cuda.synchronize()
     t_ref = time.perf_counter() - t
     refs = []
     for i in range(n):
@@ -277,60 +315,58 @@ def bench_learned_batched(args, torch, K, G):
         refs.append(r)
     torch.cuda.synchronize()
     log(f"[bench] references: 100 batched MLEM-A iterations in {t_ref:.2f} s")
-    wpath = os.path.join(ROOT, "kevernals_amd", "weights", "cnn_mlem5_to_100.npz")
-    with np.load(wpath, allow_pickle=False) as f:
-        params, scale = f["params"], float(f["scale"])
+    params, scale = _load_cnn_params("cnn_schedule_to_100.npz")
+    sch = Schedule()
     runs = {}
-    for name, prm in (("plain", None), ("learned", params)):
-        reset()
-        runs[name] = batched_gated_mlem(b, 100, params=prm, scale=scale, every=2, first=5, patience=2,
-                                        references=refs)
-
-    def time_all(run, thr):
-        for kind, k, tt, r in run.trace:
-            if r is not None and max(r) <= thr:
-                return tt, k
-        return None, None
-
-    out_t = {}
-    for thr in (0.10, 0.05, 0.02, 0.01):
-        (tp, kp), (tl, kl) = time_all(runs["plain"], thr), time_all(runs["learned"], thr)
-        out_t[f"{thr:g}"] = {"plain_s": tp, "plain_iters": kp, "learned_s": tl, "learned_iters": kl,
-                             "speedup": (tp / tl) if (tp is not None and tl) else None}
-        log(f"[bench] all {n} at RMSE <= {thr:g}: plain {tp} s ({kp} it), learned {tl} s ({kl} it)")
+    reset()
+    runs["plain"] = batched_gated_mlem(b, 100, params=None, references=refs)
+    reset()
+    runs["learned"] = batched_accelerated_mlem(b, sch, params=params, scale=scale, references=refs)
+    learned_rmse = [float(v) for v in runs["learned"].trace[-1][3]]
+    reset()
+    runs["schedule_only"] = batched_accelerated_mlem(b, sch, params=None, references=refs)
+    out_t = _thresholds(runs, lambda e: (max(e[3]) if e[3] is not None else None), f"all {n}")
     s1 = out_t["0.01"]
     out = {
         "metric": METRIC,
-        "value": round(s1["speedup"], 3) if s1["speedup"] else None,
+        "value": s1["learned_speedup"],
         "unit": "x (plain batched time / learned batched time to RMSE <= 1%)",
         "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(1e3 * (s1["learned_s"] or runs["learned"].trace[-1][2]), 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (MLEM), bf16/f32-accumulate (CNN)",
+        "dtype": "f32 (MLEM, jumps with FP64 Gram sums), bf16/f32-accumulate (CNN)",
         "data": "synthetic (seeded phantoms + 200 microcalcifications, Poisson I0=5e3)",
         "config": {"workload": "learned plus batched MLEM-A, time to RMSE<=1% vs 100-iteration MLEM",
-                   "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews, "cnn_every": 2,
-                   "cnn_first": 5, "parallelism": "single GPU"},
+                   "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews,
+                   "schedule": {"start": sch.start, "first": sch.first, "mid": sch.mid, "second": sch.second,
+                                "horizon": sch.horizon, "mlem_iterations": sch.cost},
+                   "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
+                   "parallelism": "single GPU"},
         "time_to_rmse": out_t,
-        "cnn": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
+        "gate": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
         "reference_seconds": round(t_ref, 3),
-        "final_rmse": {"plain": runs["plain"].trace[-1][3], "learned": runs["learned"].trace[-1][3]},
+        "final_rmse": {"learned": learned_rmse,
+                       "schedule_only": [float(v) for v in runs["schedule_only"].trace[-1][3]]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
 
 def bench_learned(args, torch, K, G):
-    """Time to RMSE <= 1% against the 100-iteration MLEM result, plain MLEM
-    vs MLEM with the likelihood-gated CNN (candidate after iteration 5 and
-    every 2 iterations, proposals stop after 2 consecutive rejections), on
-    the full-size workload (1024x1024x64, 25 views, phantom seed 5000 with
-    200 microcalcifications, I0 = 5e3).  Both runs are mode-A MLEM on the
-    same device; times exclude the RMSE evaluations."""
-    from kevernals_amd.learned import Cnn, gated_mlem, time_to_rmse, transmission_data
+    """Time to RMSE <= 1% against the 100-iteration MLEM-A result on the
+    full-size workload (1024x1024x64, 25 views, phantom seed 5000 with 200
+    microcalcifications, I0 = 5e3): plain MLEM-A against the accelerated
+    reconstruction (the two-stretch trajectory-jump schedule, 18 MLEM
+    iterations and two likelihood-gated jumps, with the gated CNN step on
+    its result, weights cnn_schedule_to_100.npz).  Also timed: the schedule
+    without the network, and the CNN interleaved with plain MLEM (candidate
+    after iteration 5 and every 2 iterations, weights cnn_mlem5_to_100.npz).
+    All runs are mode-A MLEM on the same device; times exclude the RMSE
+    evaluations."""
+    from kevernals_amd.learned import Cnn, accelerated_mlem, gated_mlem, transmission_data
     from kevernals_amd.phantom import phantom
+    from kevernals_amd.trajectory import Schedule
 
-    wpath = os.path.join(ROOT, "kevernals_amd", "weights", "cnn_mlem5_to_100.npz")
     g = G.full_plain()
     ctx = K.Context(0, g)
     gen = torch.Generator(device="cuda")
@@ -346,7 +382,8 @@ def bench_learned(args, torch, K, G):
     torch.cuda.synchronize()
     t_ref = time.perf_counter() - t
     log(f"[bench] reference: 100 MLEM-A iterations in {t_ref:.2f} s")
-    cnn = Cnn.load(ctx, wpath)
+    cnn = Cnn(ctx, *_load_cnn_params("cnn_schedule_to_100.npz"))
+    cnn_old = Cnn(ctx, *_load_cnn_params("cnn_mlem5_to_100.npz"))
     # CNN inference alone (MFMA roofline of the dominant kernels)
     y = torch.empty_like(ref)
     cnn.forward(ref, y)
@@ -360,38 +397,43 @@ def bench_learned(args, torch, K, G):
     cnn_ms = e0.elapsed_time(e1) / 5
     flops = 2.0 * 27 * (16 + 256 + 256 + 16) * g.nvox
     del y
-    plain = gated_mlem(ctx, None, data, 100, reference=ref)
-    learned = gated_mlem(ctx, cnn, data, 100, every=2, first=5, reference=ref, patience=2)
-    out_t = {}
-    for thr in (0.10, 0.05, 0.02, 0.01):
-        tp, tl = time_to_rmse(plain.trace, thr), time_to_rmse(learned.trace, thr)
-        out_t[f"{thr:g}"] = {"plain_s": tp, "learned_s": tl,
-                             "speedup": (tp / tl) if (tp is not None and tl) else None}
-        log(f"[bench] RMSE <= {thr:g}: plain {tp} s, learned {tl} s")
+    sch = Schedule()
+    runs = {
+        "plain": gated_mlem(ctx, None, data, 100, reference=ref),
+        "learned": accelerated_mlem(ctx, data, sch, cnn=cnn, reference=ref),
+        "schedule_only": accelerated_mlem(ctx, data, sch, reference=ref),
+        "interleaved_cnn": gated_mlem(ctx, cnn_old, data, 100, every=2, first=5, reference=ref, patience=2),
+    }
+    out_t = _thresholds(runs, lambda e: e[3], "")
     s1 = out_t["0.01"]
+    lr = runs["learned"]
     out = {
         "metric": METRIC,
-        "value": round(s1["speedup"], 3) if s1["speedup"] else None,
+        "value": s1["learned_speedup"],
         "unit": "x (plain time / learned time to RMSE <= 1%)",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 0,
-        "ms_per_step": round(1e3 * (s1["learned_s"] or learned.trace[-1][2]), 2),
+        "ms_per_step": round(1e3 * (s1["learned_s"] or lr.trace[-1][2]), 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (MLEM), bf16/f32-accumulate (CNN)",
+        "dtype": "f32 (MLEM, jumps with FP64 Gram sums), bf16/f32-accumulate (CNN)",
         "data": "synthetic (seeded phantom + 200 microcalcifications, Poisson I0=5e3)",
         "config": {"workload": "learned vs plain MLEM-A, time to RMSE<=1% vs 100-iteration MLEM",
-                   "volume": [g.nx, g.ny, g.nz], "views": g.nviews, "cnn_every": 2, "cnn_first": 5,
+                   "volume": [g.nx, g.ny, g.nz], "views": g.nviews,
+                   "schedule": {"start": sch.start, "first": sch.first, "mid": sch.mid, "second": sch.second,
+                                "horizon": sch.horizon, "mlem_iterations": sch.cost},
+                   "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
                    "parallelism": "single GPU"},
         "time_to_rmse": out_t,
-        "cnn": {"accepted": learned.accepted, "rejected": learned.rejected, "ms_per_application": round(cnn_ms, 3),
+        "steps_kept": [e[0] for e in lr.trace if e[0] != "mlem"],
+        "cnn": {"ms_per_application": round(cnn_ms, 3),
                 "roofline": {"bound": "mfma", "achieved": round(flops / (cnn_ms * 1e-3) / 1e12, 1),
                              "peak": 2500.0, "unit": "TFLOP/s",
                              "frac": round(flops / (cnn_ms * 1e-3) / 2.5e15, 4)}},
         "reference_seconds": round(t_ref, 3),
-        "final_rmse": {"plain": plain.trace[-1][3], "learned": learned.trace[-1][3]},
+        "final_rmse": {k: v.trace[-1][3] for k, v in runs.items()},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -276,38 +276,43 @@ def batched_gated_mlem(batch, iters: int, params: Optional[np.ndarray] = None, s
     return run
 
 
-def _gate(ctx, data, x, candidates, ll_x=None):
-    """First candidate whose Poisson log-likelihood is not below x's: (index,
-    its log-likelihood, x's), or (None, None, x's)."""
-    if ll_x is None:
-        ll_x = ctx.log_likelihood(data, x)
-    for i, c in enumerate(candidates):
-        ll_c = ctx.log_likelihood(data, c)
-        if ll_c >= ll_x:
-            return i, ll_c, ll_x
-    return None, None, ll_x
+def _chain_gate(ctx, data, x, y, y2=None):
+    """The likelihood gate of the final steps: the jump x -> y is kept if
+    log L(y) >= log L(x); then the CNN step y -> y2 if log L(y2) >= log L(y).
+    Returns (steps kept: 0, 1 or 2, log-likelihood of the kept volume)."""
+    ll_x = ctx.log_likelihood(data, x)
+    ll_y = ctx.log_likelihood(data, y)
+    if ll_y < ll_x:
+        return 0, ll_x
+    if y2 is None:
+        return 1, ll_y
+    ll_y2 = ctx.log_likelihood(data, y2)
+    return (2, ll_y2) if ll_y2 >= ll_y else (1, ll_y)
 
 
 def accelerated_mlem(ctx: Context, data, schedule=None, cnn: Optional[Cnn] = None, iters: Optional[int] = None,
-                     reference=None, target_rmse: Optional[float] = None, x0=None) -> GatedRun:
-    """MLEM-A with the two-stretch trajectory jump (trajectory.Schedule) and,
-    with a network, a CNN step on the final jump's result.  Every step
-    outside plain MLEM is gated: the first jump, and the final candidates
-    (CNN of the final jump, then the final jump alone) replace the estimate
-    only if its Poisson log-likelihood does not decrease; a rejected first
-    jump ends the schedule (plain MLEM from there on).  `iters` MLEM
-    iterations run in total (default: the schedule's cost, so the run ends
-    with the final jump).  Trace kinds: mlem, jump+/jump- (first jump),
-    final+/final- (the final jump), cnn+ (the CNN step kept with it).  With
-    a reference the RMSE after every step is traced (excluded from the
-    times); with target_rmse the run stops once it is reached."""
+                     reference=None, target_rmse: Optional[float] = None) -> GatedRun:
+    """MLEM-A from a uniform start with the two-stretch trajectory jump
+    (trajectory.Schedule) and, with a network, a CNN step on the jump's
+    result.  After `first` iterations the estimate x_first is held while a
+    probe (first jump, `second` MLEM iterations) collects stretch B; the
+    final jump from x_first then replaces the estimate only if its Poisson
+    log-likelihood is not lower than x_first's, and the CNN step only if it
+    does not lower the jump's.  A rejected jump leaves x_first, and plain
+    MLEM continues from it.  `iters` MLEM iterations run in total (probe
+    included; default: the schedule's cost, so the run ends with the final
+    steps).  Trace kinds: mlem (an iteration of the estimate), probe (an
+    iteration of the probe; the traced RMSE is the estimate's), final+ /
+    final- (jump kept / rejected), cnn+ (the CNN step kept as well).  With a
+    reference the RMSE of the estimate after every step is traced (excluded
+    from the times); with target_rmse the run stops once it is reached."""
     from .trajectory import Jumper, Schedule
 
     torch = __import__("torch")
     sch = schedule or Schedule()
     iters = sch.cost if iters is None else iters
     g = ctx.geometry
-    x = torch.ones(g.vol_shape, device="cuda") if x0 is None else x0.clone()
+    x = torch.ones(g.vol_shape, device="cuda")
     jm = Jumper(ctx, g.vol_shape, sch)
     cand = torch.empty_like(x)
     cand2 = torch.empty_like(x) if cnn is not None else None
@@ -316,67 +321,52 @@ def accelerated_mlem(ctx: Context, data, schedule=None, cnn: Optional[Cnn] = Non
     t0 = time.perf_counter()
     paused = 0.0
 
-    def note(kind, k, ll=None):
+    def note(kind, k, est, ll=None):
         nonlocal paused
         torch.cuda.synchronize()
         t = time.perf_counter() - t0 - paused
         r = None
         if reference is not None:
             tp = time.perf_counter()
-            r = rmse(x, reference)
+            r = rmse(est, reference)
             paused += time.perf_counter() - tp
         run.trace.append((kind, k, t, r, ll))
         return r is not None and target_rmse is not None and r <= target_rmse
 
-    active = True
     if sch.start == 0:
         jm.record(x)
+    probing = False
     k = 0
     while k < iters:
         ctx.mlem_iteration(x, data, 1.0, MODE_A)
         k += 1
-        if active and (sch.start <= k <= sch.first or jm.count > jm.na):
+        if not probing and sch.start <= k <= sch.first and k < sch.cost:
             jm.record(x)
-        if note("mlem", k):
-            break
-        if active and k == sch.first:
-            jm.first_jump(cand)
-            i, ll_c, _ = _gate(ctx, data, x, [cand])
-            if i is None:
-                active = False
-                run.rejected += 1
-                if note("jump-", k):
-                    break
+        if probing:
+            jm.record(x)
+            if k < sch.cost:
+                note("probe", k, jm.estimate)
                 continue
-            x.copy_(cand)
-            jm.record(x)
-            run.accepted += 1
-            if note("jump+", k, ll_c):
-                break
-        elif active and k == sch.cost:
-            active = False
+            # final steps: the estimate is x_first
+            probing = False
             jm.final_jump(cand)
-            cands = [cand]
             if cnn is not None:
                 cnn.forward(cand, cand2)
-                cands = [cand2, cand]
-            i, ll_c, _ = _gate(ctx, data, x, cands)
-            if i is None:
-                run.rejected += 1
-                if note("final-", k):
-                    break
-                continue
-            x.copy_(cands[i])
-            run.accepted += 1
-            if cnn is not None and i == 0:
-                run.accepted += 1
-                done = note("cnn+", k, ll_c)
-            else:
-                if cnn is not None:
-                    run.rejected += 1
-                done = note("final+", k, ll_c)
-            if done:
+            kept, ll = _chain_gate(ctx, data, jm.estimate, cand, cand2)
+            x.copy_((jm.estimate, cand, cand2)[kept])
+            run.accepted += kept
+            run.rejected += (2 if cnn is not None else 1) - kept
+            if note(("final-", "final+", "cnn+")[kept], k, x, ll):
                 break
+            continue
+        if note("mlem", k, x):
+            break
+        if k == sch.first and k < sch.cost:
+            # the probe: x becomes the jump to iteration `mid`
+            jm.first_jump(cand)
+            x.copy_(cand)
+            jm.record(x)
+            probing = True
     run.x = x
     return run
 
@@ -384,12 +374,15 @@ def accelerated_mlem(ctx: Context, data, schedule=None, cnn: Optional[Cnn] = Non
 def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] = None,
                              scale: float = DEFAULT_SCALE, iters: Optional[int] = None,
                              references=None) -> BatchedRun:
-    """The two-stretch jump schedule on every reconstruction of a mode-A batch:
-    the MLEM iterations of all reconstructions run as one persistent forward
-    and one persistent backprojection launch per iteration (captured
-    hipGraph), each reconstruction records its own stretches and makes its
-    own gated jumps (and CNN step, with params) through its batch context.
-    Volumes start from the batch's current contents."""
+    """The two-stretch jump schedule on every reconstruction of a mode-A batch
+    (as accelerated_mlem, reconstruction by reconstruction): the MLEM
+    iterations of all reconstructions (estimates and probes alike) run as
+    one persistent forward and one persistent backprojection launch per
+    iteration (captured hipGraph); each reconstruction records its own
+    stretches straight from the batch's volume buffers and makes its own
+    jumps and gated final steps through its batch context.  Volumes start
+    from the batch's current contents.  Trace entries: (kind, iteration,
+    seconds, [RMSE of each reconstruction's estimate] or None)."""
     from .trajectory import Jumper, Schedule
 
     torch = __import__("torch")
@@ -404,11 +397,7 @@ def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] 
     cand = [torch.empty(s, device="cuda") for s in shapes]
     cand2 = [torch.empty(s, device="cuda") for s in shapes] if cnn is not None else None
     jms = [Jumper(ctxs[i], shapes[i], sch) for i in range(n)]
-    active = [True] * n
     run = BatchedRun(accepted=[0] * n, rejected=[0] * n)
-
-    def fetch(i):
-        ctxs[i].copy(views[i], bufs[i][0], views[i].numel() * 4)
 
     def store(i, src):
         ctxs[i].copy(bufs[i][0], src, src.numel() * 4)
@@ -417,7 +406,7 @@ def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] 
     t0 = time.perf_counter()
     paused = 0.0
 
-    def note(kind, k):
+    def note(kind, k, probing):
         nonlocal paused
         torch.cuda.synchronize()
         t = time.perf_counter() - t0 - paused
@@ -426,56 +415,48 @@ def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] 
             tp = time.perf_counter()
             r = []
             for i in range(n):
-                fetch(i)
-                r.append(rmse(views[i], references[i]))
+                if probing:
+                    r.append(rmse(jms[i].estimate, references[i]))
+                else:
+                    ctxs[i].copy(views[i], bufs[i][0], views[i].numel() * 4)
+                    r.append(rmse(views[i], references[i]))
             torch.cuda.synchronize()
             paused += time.perf_counter() - tp
         run.trace.append((kind, k, t, r))
 
     if sch.start == 0:
         for i in range(n):
-            fetch(i)
-            jms[i].record(views[i])
+            jms[i].record(bufs[i][0])
+    probing = False
     k = 0
     while k < iters:
         batch.mlem(1, use_graph=True)
         k += 1
-        for i in range(n):
-            if active[i] and (sch.start <= k <= sch.first or jms[i].count > jms[i].na):
-                fetch(i)
-                jms[i].record(views[i])
-        note("mlem", k)
-        if k == sch.first or k == sch.cost:
+        if probing or (sch.start <= k <= sch.first and k < sch.cost):
+            # copies on the contexts' stream; note() synchronises the device
+            # before the next batched launch writes these buffers
             for i in range(n):
-                if not active[i]:
-                    continue
-                fetch(i)
-                data_addr = bufs[i][1]
-                if k == sch.first:
-                    jms[i].first_jump(cand[i])
-                    cands = [cand[i]]
-                else:
-                    active[i] = False
-                    jms[i].final_jump(cand[i])
-                    cands = [cand[i]]
-                    if cnn is not None:
-                        cnn.forward(cand[i], cand2[i])
-                        cands = [cand2[i], cand[i]]
-                j, _, _ = _gate(ctxs[i], data_addr, views[i], cands)
-                if j is None:
-                    active[i] = False
-                    run.rejected[i] += 1
-                    continue
-                store(i, cands[j])
-                run.accepted[i] += 1
-                if k == sch.first:
-                    jms[i].record(cands[j])
-                elif cnn is not None:
-                    # the CNN candidate kept (j == 0) or passed over for the jump alone
-                    if j == 0:
-                        run.accepted[i] += 1
-                    else:
-                        run.rejected[i] += 1
+                jms[i].record(bufs[i][0])
+        if probing and k == sch.cost:
+            probing = False
+            for i in range(n):
+                jms[i].final_jump(cand[i])
+                if cnn is not None:
+                    cnn.forward(cand[i], cand2[i])
+                kept, _ = _chain_gate(ctxs[i], bufs[i][1], jms[i].estimate, cand[i],
+                                      cand2[i] if cnn is not None else None)
+                store(i, (jms[i].estimate, cand[i], cand2[i] if cnn is not None else None)[kept])
+                run.accepted[i] += kept
+                run.rejected[i] += (2 if cnn is not None else 1) - kept
             torch.cuda.synchronize()  # the batch stream must see the new volumes
-            note("jump" if k == sch.first else "final", k)
+            note("final", k, False)
+            continue
+        note("probe" if probing else "mlem", k, probing)
+        if k == sch.first and k < sch.cost:
+            for i in range(n):
+                jms[i].first_jump(cand[i])
+                store(i, cand[i])
+                jms[i].record(cand[i])
+            torch.cuda.synchronize()
+            probing = True
     return run

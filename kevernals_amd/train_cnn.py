@@ -7,6 +7,9 @@ paired with its 100-iteration estimate).  --input krylov pairs the
 trajectory-extrapolated estimate made at iteration --at (from the last
 --history log-ratios, kevernals_amd.trajectory) with the 100-iteration
 estimate instead: the network then corrects what the extrapolation misses.
+--input schedule pairs the final estimate of the two-stretch jump schedule
+(kevernals_amd.trajectory.Schedule: 18 MLEM iterations, two jumps) with the
+100-iteration estimate.
 
     python -m kevernals_amd.train_cnn --pairs 2000 --epochs 4 --out weights.npz
 
@@ -73,6 +76,46 @@ def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref, gen):
     return xs, ts
 
 
+def make_schedule_pairs(ctx, seeds, pool, i0, k_ref, gen):
+    """(input, target) pairs: the final estimate of the jump schedule (jumps
+    not gated) and the k_ref-iteration estimate of the same data."""
+    import torch
+
+    from kevernals_amd.learned import transmission_data
+    from kevernals_amd.trajectory import Jumper, Schedule
+
+    g = ctx.geometry
+    sch = Schedule(horizon=k_ref)
+    xs, ts = [], []
+    t0 = time.time()
+    for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
+        mu = torch.from_numpy(mu_h).cuda()
+        data = transmission_data(ctx, mu, i0, gen)
+        jm = Jumper(ctx, g.vol_shape, sch)
+        x = torch.ones(g.vol_shape, device="cuda")
+        if sch.start == 0:
+            jm.record(x)
+        for k in range(1, k_ref + 1):
+            ctx.mlem_iteration(x, data, 1.0, 1)
+            if sch.start <= k <= sch.first:
+                jm.record(x)
+            if k == sch.first:
+                xb = torch.empty_like(x)
+                jm.first_jump(xb)
+                jm.record(xb)
+                for _ in range(sch.second):
+                    ctx.mlem_iteration(xb, data, 1.0, 1)
+                    jm.record(xb)
+                jm.final_jump(xb)
+                xs.append(xb)
+                del jm
+        ts.append(x)
+        if (n + 1) % 50 == 0 or n + 1 == len(seeds):
+            torch.cuda.synchronize()
+            log(f"[train] pairs {n + 1}/{len(seeds)} in {time.time() - t0:.1f} s")
+    return xs, ts
+
+
 def make_pairs(ctx, seeds, pool, i0, stages, k_ref, gen):
     """(input, target) pairs: the MLEM-A estimate after each iteration count
     in `stages`, paired with the estimate after k_ref iterations of the same
@@ -117,7 +160,7 @@ def main(argv=None):
     ap.add_argument("--stages", default=None,
                     help="comma-separated input iteration counts (default: --k-in only)")
     ap.add_argument("--k-ref", type=int, default=100)
-    ap.add_argument("--input", choices=["mlem", "krylov"], default="mlem")
+    ap.add_argument("--input", choices=["mlem", "krylov", "schedule"], default="mlem")
     ap.add_argument("--at", type=int, default=20, help="--input krylov: iteration of the jump")
     ap.add_argument("--history", type=int, default=18, help="--input krylov: log-ratios used")
     ap.add_argument("--workers", type=int, default=8)
@@ -147,6 +190,10 @@ def main(argv=None):
         xs, ts = make_krylov_pairs(ctx, train_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
         vx, vt = make_krylov_pairs(ctx, val_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
         stages = [args.at]
+    elif args.input == "schedule":
+        xs, ts = make_schedule_pairs(ctx, train_seeds, pool, args.i0, args.k_ref, gen)
+        vx, vt = make_schedule_pairs(ctx, val_seeds, pool, args.i0, args.k_ref, gen)
+        stages = [0]
     else:
         xs, ts = make_pairs(ctx, train_seeds, pool, args.i0, stages, args.k_ref, gen)
         vx, vt = make_pairs(ctx, val_seeds, pool, args.i0, [args.k_in], args.k_ref, gen)

@@ -158,13 +158,15 @@ class Trajectory:
 @dataclass
 class Schedule:
     """Two-stretch jump schedule.  MLEM-A runs `first` iterations, recording
-    the iterates from `start` on (stretch A); the first jump takes the
-    estimate to iteration `mid`; `second` MLEM iterations run from there
+    the iterates from `start` on (stretch A); a probe then jumps to an
+    estimate of iteration `mid` and runs `second` MLEM iterations from there
     (stretch B: their log-ratios are filtered by about mid powers of J, so
     they span the slow modes that stretch A cannot resolve); the final jump
     starts again from iteration `first`, with the Ritz basis of both
     stretches, and lands on an estimate of iteration `horizon`.  Cost: first
-    + second MLEM iterations, two Gram passes and two combinations."""
+    + second MLEM iterations, two Gram passes and two combinations.  The
+    probe is side work: the estimate stays at iteration `first` until the
+    final jump replaces it."""
     start: int = 2
     first: int = 10
     mid: int = 40
@@ -201,10 +203,19 @@ class Jumper:
         self.count = 0
 
     def record(self, x):
+        """Copy the next iterate (a device tensor or address of a volume of
+        this geometry) into its slot, on the context's stream."""
         if self.count >= len(self.slots):
             raise RuntimeError("both stretches are recorded already")
-        self.ctx.copy(self.slots[self.count], x, x.numel() * 4)
+        dst = self.slots[self.count]
+        self.ctx.copy(dst, x, dst.numel() * 4)
         self.count += 1
+
+    @property
+    def estimate(self):
+        """Stretch A's last iterate (iteration `first`): the estimate while
+        stretch B probes ahead."""
+        return self.slots[self.na - 1]
 
     def first_jump(self, out) -> Extrapolation:
         """out = estimate of iteration `mid` from stretch A (complete)."""

@@ -48,23 +48,26 @@ def test_schedule_beats_plain_mlem_at_equal_cost(K):
     ctx, data = problem(K, g, 8100)
     x = torch.ones(g.vol_shape, device="cuda")
     plain = {}
+    sch = Schedule()
     for k in range(1, 101):
         ctx.mlem_iteration(x, data, 1.0, K.MODE_A)
-        if k in (18, 40, 80):
+        if k in (sch.first, 18, 40, 80):
             plain[k] = x.clone()
     ref = x
-    sch = Schedule()
     run = accelerated_mlem(ctx, data, sch, reference=ref)
     kinds = [t[0] for t in run.trace]
-    assert kinds.count("mlem") == sch.cost and "jump+" in kinds and "final+" in kinds
+    assert kinds == ["mlem"] * sch.first + ["probe"] * (sch.second - 1) + ["final+"]
+    assert [t[1] for t in run.trace] == list(range(1, sch.cost + 1))
+    # while the probe runs, the estimate is iteration `first`
+    r_first = rmse(plain[sch.first], ref)
+    assert all(t[3] == r_first for t in run.trace if t[0] == "probe")
     r_acc = run.trace[-1][3]
     r18, r40, r80 = (rmse(plain[k], ref) for k in (18, 40, 80))
     print(f"[accel] RMSE vs iteration 100: schedule ({sch.cost} iterations) {r_acc:.4f}; plain 18 {r18:.4f}, "
           f"40 {r40:.4f}, 80 {r80:.4f}", flush=True)
     assert r_acc < r18 / 5 and r_acc < r40
-    # the gate's record: log-likelihood never decreased at a kept step
-    lls = [t[4] for t in run.trace if t[4] is not None]
-    assert lls == sorted(lls)
+    # the gate: the kept jump does not lower the likelihood of iteration `first`
+    assert run.trace[-1][4] >= ctx.log_likelihood(data, plain[sch.first])
 
 
 def test_harmful_network_is_rejected(K):
@@ -77,8 +80,7 @@ def test_harmful_network_is_rejected(K):
     bad = init_params(3)
     bad[-1] = 5.0  # last-layer bias: a large constant offset
     run = accelerated_mlem(ctx, data, cnn=Cnn(ctx, bad))
-    kinds = [t[0] for t in run.trace]
-    assert "final+" in kinds and "cnn+" not in kinds
+    assert run.trace[-1][0] == "final+" and run.accepted == 1 and run.rejected == 1
     np.testing.assert_array_equal(run.x.cpu().numpy(), plain.x.cpu().numpy())
 
 
@@ -94,7 +96,7 @@ def test_batched_schedule_is_bitwise_unbatched(K):
              ((64, 64, 24), (0.5, 0.5, 1.5), (128, 128), (0.5, 0.5), 5, 12.0)]
     gs = [G.arc_geometry(*s) for s in specs]
     probs = [problem(K, g, 8300 + i) for i, g in enumerate(gs)]
-    wpath = os.path.join(os.path.dirname(K.__file__), "weights", "cnn_mlem5_to_100.npz")
+    wpath = os.path.join(os.path.dirname(K.__file__), "weights", "cnn_schedule_to_100.npz")
     with np.load(wpath, allow_pickle=False) as f:
         params, scale = f["params"], float(f["scale"])
     b = K.Batch(gs, mode=K.MODE_A)
@@ -106,3 +108,29 @@ def test_batched_schedule_is_bitwise_unbatched(K):
         run = accelerated_mlem(ctx, data, cnn=Cnn(ctx, params, scale))
         np.testing.assert_array_equal(b.get_volume(i), run.x.cpu().numpy(), err_msg=f"problem {i}")
         assert brun.accepted[i] == run.accepted and brun.rejected[i] == run.rejected
+
+
+def test_trained_network_gains_over_the_schedule(K):
+    """Held-out phantoms (seeds 6100-6102, not among the training seeds
+    4000-5999) at the training geometry (256x256x32, 15 views, I0 = 1e4):
+    the network trained on the schedule's final estimates passes the gate
+    (its step does not lower the likelihood of the jump's result) and lowers
+    the RMSE against the 100-iteration MLEM-A result."""
+    import os
+
+    from kevernals_amd import geometry as G
+    from kevernals_amd.learned import Cnn, accelerated_mlem, rmse
+
+    g = G.learned_train()
+    wpath = os.path.join(os.path.dirname(K.__file__), "weights", "cnn_schedule_to_100.npz")
+    for seed in (6100, 6101, 6102):
+        ctx, data = problem(K, g, seed)
+        x = torch.ones(g.vol_shape, device="cuda")
+        for _ in range(100):
+            ctx.mlem_iteration(x, data, 1.0, K.MODE_A)
+        alone = accelerated_mlem(ctx, data)
+        learned = accelerated_mlem(ctx, data, cnn=Cnn.load(ctx, wpath))
+        r0, r1 = rmse(alone.x, x), rmse(learned.x, x)
+        print(f"[accel] seed {seed}: schedule {r0:.5f}, schedule + CNN {r1:.5f} ({learned.trace[-1][0]})", flush=True)
+        assert learned.trace[-1][0] == "cnn+"
+        assert r1 < r0
