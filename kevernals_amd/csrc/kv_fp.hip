@@ -246,13 +246,6 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     }
 }
 
-// Voxel gather with a 32-bit byte offset from the (wave-uniform) volume base:
-// volumes are limited to 2^30 voxels, so k << 2 cannot wrap.
-__device__ __forceinline__ float ld_voxel(const float *__restrict__ base, unsigned k)
-{
-    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (k << 2));
-}
-
 // AlmostEqualRelative(v, 0) of commons/Maths.h:25-34 in FP32.
 __device__ __forceinline__ bool almost_equal_zero(float a)
 {
@@ -396,7 +389,27 @@ __device__ __forceinline__ float plane_a(const RayA &r, int c, int k)
     return (((float)k - r.h[c]) - r.s[c]) * r.inv[c];
 }
 
-// Mode-A Siddon walk: visit(voxel, len) for every voxel with len > 0.
+// Maximum of three values as one v_max3_f32 (operands finite or -INF, never
+// NaN: no canonicalisation needed).
+__device__ __forceinline__ float max3f(float a, float b, float c)
+{
+    float m;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(a), "v"(b), "v"(c));
+    return m;
+}
+
+// Mode-A Siddon walk: visit(ok, voxel, len) for every voxel the ray's
+// parameter interval [ain, aout] passes through, ok = len > 0 (voxel is the
+// linear index; in range for every call).
+// Each axis keeps the crossing of its next plane, hi = ((k - N/2) - s) / d
+// evaluated as plane_a does, with k - N/2 carried as an exact float (an
+// integer or half-integer far below 2^24), the crossing of the last plane
+// passed (lo) and the voxel index folded into one linear index.  The axis
+// that exits first (first minimum on ties, as the oracle) is found from the
+// minimum of the three heads; an axis without motion holds hi = +INF.  The
+// walk ends when the nearest exit reaches aout: the exit through an axis's
+// boundary plane is that plane's crossing, which is where aout comes from,
+// so no index is ever stepped outside the volume.
 template <class Visit>
 __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
 {
@@ -415,22 +428,21 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
     if (!(ain < aout))
         return;
     int ii[3];
-    float lo[3], hi[3];
-    int stp[3];
+    float lo[3], hi[3], kf[3], st[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         if (r.d[c] == 0.f) {
             ii[c] = r.fixed[c];
             lo[c] = -kInf;
             hi[c] = kInf;
-            stp[c] = 0;
+            kf[c] = 0.f;
+            st[c] = 0.f;
             continue;
         }
         const float pos = r.s[c] + r.h[c] + ain * r.d[c];
         int i = (int)floorf(pos);
         i = max(0, min(r.n[c] - 1, i));
-        const bool up = r.d[c] > 0.f;
-        stp[c] = up ? 1 : -1;
+        const int stp = r.d[c] > 0.f ? 1 : -1;
         // lo/hi of voxel i on axis c
         auto lohi = [&](int k, float &l, float &h) {
             const float a0 = plane_a(r, c, k), a1 = plane_a(r, c, k + 1);
@@ -440,51 +452,46 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
         float l, h;
         for (;;) {
             lohi(i, l, h);
-            const int j = i - stp[c];
+            const int j = i - stp;
             if (l > ain && j >= 0 && j < r.n[c]) i = j; else break;
         }
         for (;;) {
             lohi(i, l, h);
-            const int j = i + stp[c];
+            const int j = i + stp;
             if (h <= ain && j >= 0 && j < r.n[c]) i = j; else break;
         }
         ii[c] = i;
         lo[c] = l;
         hi[c] = h;
+        // the plane whose crossing is hi: k = i + 1 going up, i going down
+        kf[c] = (float)(stp > 0 ? i + 1 : i) - r.h[c];
+        st[c] = (float)stp;
     }
+    const int nx = r.n[0], nxy = r.n[0] * r.n[1];
+    const int sx = (int)st[0], sy = (int)st[1] * nx, sz = (int)st[2] * nxy;
+    unsigned idx = (unsigned)(ii[2] * nxy + ii[1] * nx + ii[0]);
     const int guard = r.n[0] + r.n[1] + r.n[2] + 3;
-    const unsigned nx = (unsigned)r.n[0], ny = (unsigned)r.n[1];
     for (int step = 0; step < guard; ++step) {
-        const float top = fminf(fminf(1.f, hi[0]), fminf(hi[1], hi[2]));
-        const float bot = fmaxf(fmaxf(0.f, lo[0]), fmaxf(lo[1], lo[2]));
-        const float len = top - bot;
-        if (len > 0.f)
-            visit(((unsigned)ii[2] * ny + (unsigned)ii[1]) * nx + (unsigned)ii[0], len);
-        // axis with the nearest exit (first minimum on ties, as the oracle)
-        int m = -1;
-        float best = kInf;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (stp[c] != 0 && hi[c] < best) {
-                best = hi[c];
-                m = c;
-            }
-        if (m < 0 || best >= aout)
+        const float best = min3f(hi[0], hi[1], hi[2]);
+        const float len = fminf(1.f, best) - fmaxf(0.f, max3f(lo[0], lo[1], lo[2]));
+        visit(len > 0.f, idx, len);
+        if (best >= aout)
             break;
-        // static indices only (no dynamically indexed register arrays)
-        bool leave = false;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c == m) {
-                ii[c] += stp[c];
-                const int k = stp[c] > 0 ? ii[c] + 1 : ii[c];
-                lo[c] = hi[c];
-                hi[c] = plane_a(r, c, k);
-                leave = ii[c] < 0 || ii[c] >= r.n[c];
-            }
-        }
-        if (leave)
-            break;
+        const bool e0 = hi[0] == best;
+        const bool e1 = !e0 && hi[1] == best;
+        const bool e2 = !e0 && !e1;
+        idx += (unsigned)(e0 ? sx : (e1 ? sy : sz));
+        // the exiting axis: its next plane (same operation sequence as plane_a)
+        const float k0 = kf[0] + st[0], k1 = kf[1] + st[1], k2 = kf[2] + st[2];
+        lo[0] = e0 ? hi[0] : lo[0];
+        lo[1] = e1 ? hi[1] : lo[1];
+        lo[2] = e2 ? hi[2] : lo[2];
+        hi[0] = e0 ? (k0 - r.s[0]) * r.inv[0] : hi[0];
+        hi[1] = e1 ? (k1 - r.s[1]) * r.inv[1] : hi[1];
+        hi[2] = e2 ? (k2 - r.s[2]) * r.inv[2] : hi[2];
+        kf[0] = e0 ? k0 : kf[0];
+        kf[1] = e1 ? k1 : kf[1];
+        kf[2] = e2 ? k2 : kf[2];
     }
 }
 
@@ -498,12 +505,16 @@ __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const float *__restric
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
     if (EP == KV_EP_COUNT) {
         int c = 0;
-        walk_a(r, [&](unsigned, float) { ++c; });
+        walk_a(r, [&](bool ok, unsigned, float) { c += ok; });
         out[o] = (float)c;
         return;
     }
     float acc = 0.f;
-    walk_a(r, [&](unsigned k, float len) { acc += len * ld_voxel(vol, k); });
+    const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
+    walk_a(r, [&](bool ok, unsigned k, float len) {
+        const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
+        acc = ok ? acc + len * val : acc;
+    });
     const float ax = r.L * acc;
     if (EP == KV_EP_PLAIN)
         out[o] = ax;
@@ -582,7 +593,7 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
     } else {
         RayA r;
         ray_setup_a(g, p, u, v, r);
-        walk_a(r, [&](unsigned k, float wt) { rec(true, k, wt); });
+        walk_a(r, rec);
     }
     counts[t] = c;
 }
