@@ -109,7 +109,9 @@ kv_status kv_mlem_iteration(kv_context *ctx, float *d_vol, const float *d_data, 
  * work or time the passes separately: kv_mlem_forward writes the ratio
  * (Reconstructors.h:185,207-213) into the context's work buffer,
  * kv_mlem_backward backprojects it and applies the update
- * (Reconstructors.h:225,238-241). */
+ * (Reconstructors.h:225,238-241).  Other calls that use the work buffer
+ * (kv_back_project in mode A, kv_log_likelihood, kv_sart_iteration,
+ * kv_count_pairs) between the two overwrite the ratio. */
 kv_status kv_mlem_forward(kv_context *ctx, const float *d_vol, const float *d_data, float relax, int mode);
 kv_status kv_mlem_backward(kv_context *ctx, float *d_vol, int mode);
 /* One iteration of recons::ART (Reconstructors.h:65-122), which is a

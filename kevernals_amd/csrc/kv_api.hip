@@ -379,7 +379,7 @@ kv_status kv_forward_project(kv_context *ctx, const float *d_vol, float *d_proj,
     if (mode == KV_MODE_R)
         KV_TRY_HIP(kv_launch_fp_r(ctx->g, d_vol, d_proj, KV_EP_PLAIN, nullptr, 0.f, ctx->stream));
     else
-        KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, d_proj, KV_EP_PLAIN, nullptr, ctx->stream));
+        KV_TRY_HIP(kv_launch_fp_a(ctx->g, ctx->tab_a, d_vol, d_proj, KV_EP_PLAIN, nullptr, ctx->stream));
     return KV_OK;
 }
 
@@ -395,10 +395,14 @@ kv_status kv_back_project(kv_context *ctx, const float *d_proj, float *d_vol, in
     }
     if (kv_status st = ensure_tables(ctx, mode))
         return st;
-    if (mode == KV_MODE_R)
+    if (mode == KV_MODE_R) {
         KV_TRY_HIP(kv_launch_bp_r(ctx->g, ctx->tab_r, d_proj, d_vol, KV_EP_PLAIN, ctx->stream));
-    else
-        KV_TRY_HIP(kv_launch_bp_a(ctx->g, ctx->tab_a, d_proj, d_vol, KV_EP_PLAIN, nullptr, false, ctx->stream));
+    } else {
+        // each ray's length factor is applied once per pixel, into the
+        // projection-sized work buffer, not once per (voxel, pixel) pair
+        KV_TRY_HIP(kv_launch_scale_a(ctx->g, ctx->tab_a, d_proj, ctx->d_ratio, ctx->stream));
+        KV_TRY_HIP(kv_launch_bp_a(ctx->g, ctx->tab_a, ctx->d_ratio, d_vol, KV_EP_PLAIN, nullptr, false, ctx->stream));
+    }
     return KV_OK;
 }
 
@@ -419,7 +423,7 @@ kv_status kv_mlem_forward(kv_context *ctx, const float *d_vol, const float *d_da
     if (mode == KV_MODE_R)
         KV_TRY_HIP(kv_launch_fp_r(ctx->g, d_vol, ctx->d_ratio, KV_EP_MLEM, d_data, relax, ctx->stream));
     else
-        KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, ctx->d_ratio, KV_EP_MLEM, d_data, ctx->stream));
+        KV_TRY_HIP(kv_launch_fp_a(ctx->g, ctx->tab_a, d_vol, ctx->d_ratio, KV_EP_MLEM, d_data, ctx->stream));
     return KV_OK;
 }
 
@@ -480,7 +484,7 @@ kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *
         const int64_t n = pass == 0 ? kv_nray(ctx->g) : kv_nvox(ctx->g);
         if (pass == 0)
             e = mode == KV_MODE_R ? kv_launch_fp_r(ctx->g, nullptr, buf, KV_EP_COUNT, nullptr, 0.f, ctx->stream)
-                                  : kv_launch_fp_a(ctx->g, nullptr, buf, KV_EP_COUNT, nullptr, ctx->stream);
+                                  : kv_launch_fp_a(ctx->g, ctx->tab_a, nullptr, buf, KV_EP_COUNT, nullptr, ctx->stream);
         else
             e = mode == KV_MODE_R ? kv_launch_bp_r(ctx->g, ctx->tab_r, nullptr, buf, KV_EP_COUNT, ctx->stream)
                                   : kv_launch_bp_a(ctx->g, ctx->tab_a, nullptr, buf, KV_EP_COUNT, nullptr, true,
@@ -612,7 +616,7 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
             return st;
         ctx->red_cap = nb;
     }
-    KV_TRY_HIP(kv_launch_fp_a(ctx->g, d_vol, ctx->d_ratio, KV_EP_PLAIN, nullptr, ctx->stream));
+    KV_TRY_HIP(kv_launch_fp_a(ctx->g, ctx->tab_a, d_vol, ctx->d_ratio, KV_EP_PLAIN, nullptr, ctx->stream));
     KV_TRY_HIP(kv_launch_loglik(ctx->g, d_data, ctx->d_ratio, ctx->d_red, nb, ctx->stream));
     std::vector<double> part(nb);
     KV_TRY_HIP(hipMemcpyAsync(part.data(), ctx->d_red, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -435,7 +435,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
                             Lx[c] = bmax(lzp, bmin(a1x, a2x));
                             Hx[c] = bmin(uz, bmax(a1x, a2x));
-                            if (MODE == KV_MODE_A)
+                            if (MODE == KV_MODE_A && UNIT)
                                 L2[c] = lx2[lx + c];
                         }
                     }
@@ -462,7 +462,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                         // max/min are exact: the reference's grouping
                         // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
                         const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
-                        const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
+                        const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
                         off += 4 * g.nu;
                         if (EP != KV_EP_COUNT && !UNIT) {
 #pragma unroll
@@ -480,12 +480,13 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             } else if (MODE == KV_MODE_R) {
                                 wsm = ok ? wsm + w : wsm;
                                 a = ok ? a + w * v[c] : a;
-                            } else {
+                            } else if (UNIT) {
                                 const float L = sqrtf((L2[c] + lyy) + lz2);
-                                if (UNIT)
-                                    a = ok ? a + w * L : a;
-                                else
-                                    a = ok ? a + w * (L * v[c]) : a;
+                                a = ok ? a + w * L : a;
+                            } else {
+                                // input already multiplied by the ray's
+                                // length factor (kv_launch_scale_a)
+                                a = ok ? a + w * v[c] : a;
                             }
                         }
 #pragma unroll
@@ -513,7 +514,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                     const float lo = fmaxf(lz, fminf(a1y, a2y));
                     const float up = fminf(uz, fmaxf(a1y, a2y));
                     const float *prow = pr + (size_t)py * g.nu;
-                    const float lyy = MODE == KV_MODE_A ? ly2[py] : 0.f;
+                    const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
                     for (int px = lx; px <= hxw; ++px) {
                         const int j0 = px - xb0, j1 = px - xb1;
                         const float a1x = (j0 >= 0 && j0 < XBW) ? xr0[j0] : tx[(size_t)x * g.nu + px];
@@ -525,12 +526,10 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             } else if (MODE == KV_MODE_R) {
                                 wsm += w;
                                 a += w * prow[px];
+                            } else if (UNIT) {
+                                a += w * sqrtf((lx2[px] + lyy) + lz2);
                             } else {
-                                const float L = sqrtf((lx2[px] + lyy) + lz2);
-                                if (UNIT)
-                                    a += w * L;
-                                else
-                                    a += w * (L * prow[px]);
+                                a += w * prow[px];
                             }
                         }
                     }
@@ -567,6 +566,25 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
 }
 
 constexpr int kZT = 8;
+
+// out[p][v][u] = proj[p][v][u] * sqrt((lx2[p][u] + ly2[p][v]) + lz2[p]): each
+// ray's length factor, applied once per pixel.  The mode-A backprojector's
+// weight of pixel (u, v) for a voxel is the alpha-length times this factor
+// times the pixel value; the product factor * value is the same FP32 value
+// whether formed here or per (voxel, pixel) pair.
+__global__ __launch_bounds__(256) void scale_a_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
+                                                      float *__restrict__ out)
+{
+    const int64_t plane = (int64_t)g.nu * g.nv;
+    const int64_t n = plane * g.np;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(i / plane);
+        const int64_t r = i - p * plane;
+        const int v = (int)(r / g.nu), u = (int)(r - (int64_t)v * g.nu);
+        const float L = sqrtf((t.lx2[(size_t)p * g.nu + u] + t.ly2[(size_t)p * g.nv + v]) + t.lz2[p]);
+        out[i] = L * proj[i];
+    }
+}
 
 // One brick per workgroup.
 template <int MODE, int EP, int ZT, bool UNIT>
@@ -679,6 +697,14 @@ hipError_t kv_launch_bp_r(const KvGeom &g, const KvTables &t, const float *proj,
     case KV_EP_COUNT: return launch_tiled<KV_MODE_R, KV_EP_COUNT, true>(g, t, nullptr, vol, nullptr, s);
     default: return launch_tiled<KV_MODE_R, KV_EP_SART, false>(g, t, proj, vol, nullptr, s);
     }
+}
+
+hipError_t kv_launch_scale_a(const KvGeom &g, const KvTables &t, const float *proj, float *out, hipStream_t s)
+{
+    const int64_t n = kv_nray(g);
+    const int nb = (int)std::min<int64_t>(8192, (n + 255) / 256);
+    scale_a_kernel<<<std::max(nb, 1), 256, 0, s>>>(g, t, proj, out);
+    return hipGetLastError();
 }
 
 hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,

@@ -497,8 +497,8 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
 
 // One ray of the mode-A forward pass with its epilogue.
 template <int EP>
-__device__ __forceinline__ void fp_a_ray(const KvGeom &g, const float *__restrict__ vol, float *__restrict__ out,
-                                         const float *__restrict__ data, int p, int u, int v)
+__device__ __forceinline__ void fp_a_ray(const KvGeom &g, const KvTables &t, const float *__restrict__ vol,
+                                         float *__restrict__ out, const float *__restrict__ data, int p, int u, int v)
 {
     RayA r;
     ray_setup_a(g, p, u, v, r);
@@ -516,22 +516,28 @@ __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const float *__restric
         acc = ok ? acc + len * val : acc;
     });
     const float ax = r.L * acc;
-    if (EP == KV_EP_PLAIN)
+    if (EP == KV_EP_PLAIN) {
         out[o] = ax;
-    else  // MLEM ratio on line-integral data
-        out[o] = ax > 0.f ? data[o] / ax : 0.f;
+    } else {
+        // MLEM ratio on line-integral data, stored premultiplied by the
+        // ray's length factor as the mode-A backprojector weighs it (its
+        // tables' sqrt((lx2 + ly2) + lz2), the same FP32 product it would
+        // form per pixel)
+        const float L = sqrtf((t.lx2[(size_t)p * g.nu + u] + t.ly2[(size_t)p * g.nv + v]) + t.lz2[p]);
+        out[o] = ax > 0.f ? L * (data[o] / ax) : 0.f;
+    }
 }
 
 template <int EP>
-__global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
-                                                   const float *__restrict__ data)
+__global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, KvTables t, const float *__restrict__ vol,
+                                                   float *__restrict__ out, const float *__restrict__ data)
 {
     const int u = blockIdx.x * 64 + (threadIdx.x & 63);
     const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int p = blockIdx.z;
     if (u >= g.nu || v >= g.nv)
         return;
-    fp_a_ray<EP>(g, vol, out, data, p, u, v);
+    fp_a_ray<EP>(g, t, vol, out, data, p, u, v);
 }
 
 // Persistent mode-A forward pass (MLEM ratio) over a batch, as
@@ -562,7 +568,7 @@ __global__ __launch_bounds__(256) void fp_a_batch_kernel(const KvProblem *__rest
         int u, v;
         fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
         if (u < P.g.nu && v < P.g.nv)
-            fp_a_ray<KV_EP_MLEM>(P.g, P.vol, P.ratio, P.data, p, u, v);
+            fp_a_ray<KV_EP_MLEM>(P.g, P.t, P.vol, P.ratio, P.data, p, u, v);
     }
 }
 
@@ -613,15 +619,16 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
     return hipGetLastError();
 }
 
-hipError_t kv_launch_fp_a(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data, hipStream_t s)
+hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, float *out, int epilogue,
+                          const float *data, hipStream_t s)
 {
     const dim3 grid((g.nu + 63) / 64, (g.nv + 3) / 4, g.np), block(256);
     if (epilogue == KV_EP_PLAIN)
-        fp_a_kernel<KV_EP_PLAIN><<<grid, block, 0, s>>>(g, vol, out, data);
+        fp_a_kernel<KV_EP_PLAIN><<<grid, block, 0, s>>>(g, t, vol, out, data);
     else if (epilogue == KV_EP_COUNT)
-        fp_a_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, vol, out, data);
+        fp_a_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, t, vol, out, data);
     else
-        fp_a_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, vol, out, data);
+        fp_a_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, t, vol, out, data);
     return hipGetLastError();
 }
 

@@ -112,16 +112,24 @@ inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 // ---- launchers (kv_fp.hip, kv_bp.hip, kv_misc.hip) ----------------------
 hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data,
                           float relax, hipStream_t s);
-hipError_t kv_launch_fp_a(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data,
-                          hipStream_t s);
+// Mode-A forward pass.  KV_EP_MLEM writes the MLEM ratio data/Ax
+// premultiplied by the ray's length factor from the mode-A tables t (the
+// input the mode-A backprojector expects); the other epilogues ignore t.
+hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, float *out, int epilogue,
+                          const float *data, hipStream_t s);
 hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx,
                            float *w, int32_t *counts, hipStream_t s);
 hipError_t kv_launch_tables_r(const KvGeom &g, const KvTables &t, hipStream_t s);
 hipError_t kv_launch_tables_a(const KvGeom &g, const KvTables &t, hipStream_t s);
 hipError_t kv_launch_bp_r(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
                           hipStream_t s);
+// Mode-A backprojection of projections already multiplied by each ray's
+// length factor (kv_launch_scale_a, or the forward pass's MLEM epilogue);
+// unit_input backprojects the length factors themselves (sensitivity).
 hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj, float *vol, int epilogue,
                           const float *sens, bool unit_input, hipStream_t s);
+// out = proj * length factor of each ray (mode-A tables), np*nv*nu values.
+hipError_t kv_launch_scale_a(const KvGeom &g, const KvTables &t, const float *proj, float *out, hipStream_t s);
 // Batched persistent passes (MLEM epilogues, mode R or A).  `grid` workgroups
 // of 256 threads claim work items from *counter (zero before the launch).
 hipError_t kv_launch_fp_batch(int mode, const KvProblem *probs, int nprob, long long total,
