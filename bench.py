@@ -427,7 +427,7 @@ def bench_learned(args, torch, K, G):
                    "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
                    "parallelism": "single GPU"},
         "time_to_rmse": out_t,
-        "steps_kept": [e[0] for e in lr.trace if e[0] != "mlem"],
+        "steps_kept": [e[0] for e in lr.trace if e[0] not in ("mlem", "probe")],
         "cnn": {"ms_per_application": round(cnn_ms, 3),
                 "roofline": {"bound": "mfma", "achieved": round(flops / (cnn_ms * 1e-3) / 1e12, 1),
                              "peak": 2500.0, "unit": "TFLOP/s",
