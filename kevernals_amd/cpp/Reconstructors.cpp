@@ -5,6 +5,7 @@
 #include <iostream>
 #include <vector>
 
+#include "Marshal.h"
 #include "kevernals_amd.h"
 
 #ifdef KV_USE_REFERENCE_TREE
@@ -51,50 +52,8 @@ std::error_code make_error_code(ReconstructorsErrorCode e) { return {(int)e, g_r
 std::error_code make_error_code(ProjectorErrorCode e) { return {(int)e, g_projector_category}; }
 #endif  // KV_USE_REFERENCE_TREE
 
-// ------------------------------------------------------------ marshalling --
+// ---------------------------------------------------------------- helpers --
 namespace {
-// Projector.cu:876-940: kernel arguments from the geometry accessors.
-struct Marshalled {
-    kv_geometry g{};
-    std::vector<float> det, src;
-};
-
-Marshalled marshal(const TomoGeometry *tg)
-{
-    Marshalled m;
-    const Size3D vs = tg->GetVolume()->GetSize3D();
-    const VoxelSpacing sp = tg->GetVolume()->GetVoxelSpacing();
-    const int np = tg->nbProjectionsRois();
-    const Size2D ds = tg->projectionsRoisSize();
-    const PixelSpacing ps = tg->projectionsPixelSpacing();
-    const std::vector<Position2D> bl = tg->projectionsRoisBottomLeftPositions();
-    const std::vector<float> sy = tg->sourcesYPositions();
-    m.det.resize((size_t)np * 3);
-    m.src.resize((size_t)np * 3);
-    for (int p = 0; p < np; ++p) {
-        m.det[3 * p + 0] = bl.at((size_t)p).x;
-        m.det[3 * p + 1] = bl.at((size_t)p).y;
-        m.det[3 * p + 2] = tg->detectorsZCommonPosition();
-        m.src[3 * p + 0] = tg->sourcesXCommonPosition();
-        m.src[3 * p + 1] = sy.at((size_t)p);
-        m.src[3 * p + 2] = tg->sourcesZCommonPosition();
-    }
-    m.g.nx = vs.x;
-    m.g.ny = vs.y;
-    m.g.nz = vs.z;
-    m.g.sx = sp.x;
-    m.g.sy = sp.y;
-    m.g.sz = sp.z;
-    m.g.nu = ds.x;
-    m.g.nv = ds.y;
-    m.g.np = np;
-    m.g.pu = ps.x;
-    m.g.pv = ps.y;
-    m.g.det = m.det.data();
-    m.g.src = m.src.data();
-    return m;
-}
-
 size_t points(vtkImageData *img)
 {
     int d[3];
@@ -162,7 +121,7 @@ bool Projector::ensureContext() const
         lastMessage_ = kv_last_error();
         return false;
     }
-    Marshalled m = marshal(m_tomoGeometry);
+    MarshalledGeometry m = MarshalGeometry(m_tomoGeometry);
     if (kv_set_geometry(c, &m.g) != KV_OK) {
         lastError_ = make_error_code(ProjectorErrorCode::InvalidGeometryError);
         lastMessage_ = kv_last_error();

@@ -10,14 +10,16 @@
 //    (TomoGeometry.cpp:345);
 //  * voxel and pixel position tables are sequential FP32 sums
 //    (TomoVolume.cpp:197-215, TomoProjectionsSet.cpp:183-194).
-// The XML reader of the reference (TomoGeometry.cpp:13-437) is not part of
-// this library: geometries are built from values.
+// A geometry is built from values, or read from the reference's XML
+// geometry file (TomoGeometry.cpp:13-437) by TomoGeometry(path), whose
+// reading rules are in TomoGeometryXml.cpp.
 #pragma once
 
 #include <cmath>
 #include <cstddef>
 #include <memory>
 #include <numeric>
+#include <string>
 #include <vector>
 
 struct Position2D {
@@ -84,9 +86,19 @@ private:
     Position3D blf_;
 };
 
+// What an XML geometry file describes, in the form the value constructor of
+// TomoGeometry takes (TomoGeometryXml.cpp).
+struct TomoGeometryDescription;
+
 class TomoGeometry {
 public:
-    // sources: per-view source positions (world mm, volume-centred frame);
+    // Reads the reference's XML geometry file (TomoGeometry.cpp:13-437);
+    // IsValid() is false and ErrorMessage() says why when the file is not a
+    // valid geometry.
+    explicit TomoGeometry(const std::string &xmlGeometryFilePath);
+
+    // sources: per-view source positions (world mm, volume-centred frame;
+    // at least one per view, extra ones only enter the x and z means);
     // detectorBottomLefts: per-view bottom-left corner of the detector region
     // read (the ROI); detectorZs: per-view detector plane heights; fulcrum:
     // the reference's Rotation/center (used by shift-and-add).
@@ -111,11 +123,13 @@ public:
         detZ_ = detectorZs.empty()
                     ? 0.f
                     : (float)std::accumulate(detectorZs.begin(), detectorZs.end(), .0) / (float)detectorZs.size();
-        valid_ = !sources.empty() && sources.size() == detectorBottomLefts.size() && volumeSize.x > 0 &&
+        valid_ = !detectorBottomLefts.empty() && sources.size() >= detectorBottomLefts.size() && volumeSize.x > 0 &&
                  volumeSize.y > 0 && volumeSize.z > 0 && detectorSize.x > 0 && detectorSize.y > 0;
     }
 
     bool IsValid() const { return valid_; }
+    const std::string &ErrorMessage() const { return error_; }
+    std::string filePath() const { return filePath_; }
     TomoVolume *GetVolume() const { return volume_.get(); }
 
     // sources
@@ -148,4 +162,37 @@ private:
     float srcX_ = 0.f, srcZ_ = 0.f, detZ_ = 0.f;
     WSize3D fulcrum_;
     bool valid_ = false;
+    std::string error_, filePath_;
+
+    explicit TomoGeometry(const TomoGeometryDescription &d);
 };
+
+struct TomoGeometryDescription {
+    bool ok = false;
+    std::string error;
+    Size3D volumeSize;
+    VoxelSpacing voxelSpacing;
+    Size2D roiSize;
+    PixelSpacing pixelSpacing;
+    std::vector<Position3D> sources;
+    std::vector<Position2D> roiBottomLefts;
+    std::vector<float> detectorZs;
+    WSize3D fulcrum;
+};
+
+// Reading rules of the reference's XML geometry file (TomoGeometryXml.cpp).
+TomoGeometryDescription ReadTomoGeometryXml(const std::string &xmlGeometryFilePath);
+
+inline TomoGeometry::TomoGeometry(const TomoGeometryDescription &d)
+    : TomoGeometry(d.volumeSize, d.voxelSpacing, d.roiSize, d.pixelSpacing, d.sources, d.roiBottomLefts,
+                   d.detectorZs, d.fulcrum)
+{
+    valid_ = valid_ && d.ok;
+    error_ = d.ok ? (valid_ ? std::string() : std::string("geometry is empty")) : d.error;
+}
+
+inline TomoGeometry::TomoGeometry(const std::string &xmlGeometryFilePath)
+    : TomoGeometry(ReadTomoGeometryXml(xmlGeometryFilePath))
+{
+    filePath_ = xmlGeometryFilePath;
+}

@@ -3,6 +3,8 @@
 // the results as raw float32 files for comparison.
 //
 //   kv_dropin_check <dir>
+//   <dir>/geom.xml : the reference's XML geometry file (TomoGeometry(path),
+//                    TomoGeometry.cpp:13-437), or else
 //   <dir>/geom.txt : nx ny nz sx sy sz nu nv np pu pv fulcrum_z, then per view
 //                    src_x src_y src_z det_x det_y det_z
 //   <dir>/vol.bin, <dir>/proj.bin : float32 volume and projections
@@ -11,6 +13,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -55,25 +58,45 @@ int main(int argc, char **argv)
         return 2;
     }
     const std::string dir = argv[1];
-    std::ifstream in(dir + "/geom.txt");
-    int nx, ny, nz, nu, nv, np;
-    float sx, sy, sz, pu, pv, fz;
-    if (!(in >> nx >> ny >> nz >> sx >> sy >> sz >> nu >> nv >> np >> pu >> pv >> fz)) {
-        std::cerr << "bad geom.txt" << std::endl;
-        return 2;
+    std::vector<Position3D> src;
+    std::vector<Position2D> bl;
+    std::vector<float> dz;
+    std::unique_ptr<TomoGeometry> gp;
+    const bool from_xml = std::ifstream(dir + "/geom.xml").good();
+    if (from_xml) {
+        gp = std::make_unique<TomoGeometry>(dir + "/geom.xml");
+        if (!gp->IsValid()) {
+            std::cerr << "bad geom.xml: " << gp->ErrorMessage() << std::endl;
+            return 2;
+        }
+    } else {
+        std::ifstream in(dir + "/geom.txt");
+        int tnx, tny, tnz, tnu, tnv, tnp;
+        float tsx, tsy, tsz, tpu, tpv, tfz;
+        if (!(in >> tnx >> tny >> tnz >> tsx >> tsy >> tsz >> tnu >> tnv >> tnp >> tpu >> tpv >> tfz)) {
+            std::cerr << "bad geom.txt" << std::endl;
+            return 2;
+        }
+        src.resize((size_t)tnp);
+        bl.resize((size_t)tnp);
+        dz.resize((size_t)tnp);
+        for (int p = 0; p < tnp; ++p) {
+            float a, b, c, d, e, f;
+            in >> a >> b >> c >> d >> e >> f;
+            src[(size_t)p] = Position3D(a, b, c);
+            bl[(size_t)p] = Position2D(d, e);
+            dz[(size_t)p] = f;
+        }
+        gp = std::make_unique<TomoGeometry>(Size3D(tnx, tny, tnz), VoxelSpacing(tsx, tsy, tsz), Size2D(tnu, tnv),
+                                            PixelSpacing(tpu, tpv), src, bl, dz, WSize3D(0.f, 0.f, tfz));
     }
-    std::vector<Position3D> src((size_t)np);
-    std::vector<Position2D> bl((size_t)np);
-    std::vector<float> dz((size_t)np);
-    for (int p = 0; p < np; ++p) {
-        float a, b, c, d, e, f;
-        in >> a >> b >> c >> d >> e >> f;
-        src[(size_t)p] = Position3D(a, b, c);
-        bl[(size_t)p] = Position2D(d, e);
-        dz[(size_t)p] = f;
-    }
-    TomoGeometry geom(Size3D(nx, ny, nz), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl, dz,
-                      WSize3D(0.f, 0.f, fz));
+    TomoGeometry &geom = *gp;
+    const Size3D vs = geom.volumeSize3D();
+    const VoxelSpacing vsp = geom.volumeVoxelSpacing();
+    const Size2D ds = geom.projectionsRoisSize();
+    const PixelSpacing psp = geom.projectionsPixelSpacing();
+    const int nx = vs.x, ny = vs.y, nz = vs.z, nu = ds.x, nv = ds.y, np = geom.nbProjectionsRois();
+    const float sx = vsp.x, sy = vsp.y, sz = vsp.z, pu = psp.x, pv = psp.y;
     auto vol = image(nx, ny, nz, sx, sy, sz);
     auto proj = image(nu, nv, np, pu, pv, 1.0);
     if (!read_raw(dir + "/vol.bin", static_cast<float *>(vol->GetScalarPointer()), (size_t)nx * ny * nz) ||
@@ -128,8 +151,11 @@ int main(int argc, char **argv)
         write_raw(dir + "/saa.bin", saa.value());
 
     // error path: one reconstructed slice (Reconstructors.h:274-279)
-    TomoGeometry flat(Size3D(nx, ny, 1), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl, dz);
-    auto bad = recons::ShiftAndAdd(&flat, proj, std::nullopt);
-    status << "saa_one_slice " << (bad.has_value() ? 0 : bad.error().value()) << std::endl;
+    if (!from_xml) {
+        TomoGeometry flat(Size3D(nx, ny, 1), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl,
+                          dz);
+        auto bad = recons::ShiftAndAdd(&flat, proj, std::nullopt);
+        status << "saa_one_slice " << (bad.has_value() ? 0 : bad.error().value()) << std::endl;
+    }
     return failures == 0 ? 0 : 1;
 }

@@ -175,3 +175,70 @@ def scaled_down(g: Geometry, f: int) -> Geometry:
         sod=float(np.hypot(g.src[0, 1], g.src[0, 2])),
         sid=float(np.hypot(g.src[0, 1], g.src[0, 2]) - g.det[0, 2]),
     )
+
+
+# ---- the reference's XML geometry files -------------------------------------
+
+def to_xml(g: Geometry, path: str, center=(0.0, 0.0, 0.0), fulcrum=(0.0, 0.0, 0.0)) -> None:
+    """Write `g` as the reference's XML geometry file
+    (source/modules/geometry/TomoGeometry.cpp:13-437): Grid/scale and
+    Grid/resolution list (z, y, x) (the reader reverses them), sources and
+    detector corners are written in a frame whose origin is `center` (the
+    reader subtracts Grid/center), the detector is g's detector (totalWidth =
+    nu * pu, pixelWidth = nu pixels) and the region read is all of it.
+    Reading the file back gives g's values except where the reference's
+    reading reduces them: the pitch becomes totalWidth / nu in FP32, and the
+    sources' x and z become their means over the views (TomoTable.cpp:4-19)."""
+    cx, cy, cz = (float(c) for c in center)
+
+    def fmt(v):
+        return repr(float(v))
+
+    def xyz(p):
+        return f"{fmt(p[0] + cx)} {fmt(p[1] + cy)} {fmt(p[2] + cz)}"
+
+    lines = ['<?xml version="1.0" encoding="UTF-8"?>', "<Geometry>",
+             f"  <Rotation><center>{fmt(fulcrum[0])} {fmt(fulcrum[1])} {fmt(fulcrum[2])}</center></Rotation>",
+             "  <Grid>",
+             f"    <center>{fmt(cx)} {fmt(cy)} {fmt(cz)}</center>",
+             f"    <scale>{fmt(g.sz)} {fmt(g.sy)} {fmt(g.sx)}</scale>",
+             f"    <resolution>{g.nz} {g.ny} {g.nx}</resolution>",
+             "  </Grid>", "  <XRay>", "    <sources>"]
+    lines += [f"      <source>{xyz(s)}</source>" for s in g.src.astype(np.float64)]
+    lines += ["    </sources>", "  </XRay>", "  <Camera>",
+              f"    <totalWidth>{fmt(np.float64(g.nu) * np.float64(g.pu))}</totalWidth>",
+              f"    <totalHeight>{fmt(np.float64(g.nv) * np.float64(g.pv))}</totalHeight>",
+              f"    <pixelWidth>{g.nu}</pixelWidth>", f"    <pixelHeight>{g.nv}</pixelHeight>",
+              "    <references>"]
+    lines += [f"      <reference>{xyz(d)}</reference>" for d in g.det.astype(np.float64)]
+    lines += ["    </references>", "  </Camera>", "  <Radios>", "    <rois>"]
+    lines += [f"      <roi>0 0 0 0 {g.nu - 1} {g.nv - 1}</roi>" for _ in range(g.nviews)]
+    lines += ["    </rois>", "  </Radios>", "</Geometry>", ""]
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def from_xml(path: str) -> Geometry:
+    """Read the reference's XML geometry file into the kernel arguments its
+    projector would launch with (TomoGeometry.cpp:13-437 read by
+    kevernals_amd/cpp/TomoGeometryXml.cpp, gathered as Projector.cu:876-940).
+    Raises ValueError with the reason when the file is not a valid geometry."""
+    import ctypes
+    import os
+
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libkevernals_dropin.so"))
+    fn = lib.kvd_geometry_from_xml
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    ints = np.zeros(7, np.int32)
+    floats = np.zeros(8, np.float32)
+    err = ctypes.create_string_buffer(512)
+    np_ = fn(os.fsencode(path), ints.ctypes.data, floats.ctypes.data, None, None, 0, err, 512)
+    if np_ <= 0:
+        raise ValueError(f"{path}: {err.value.decode(errors='replace')}")
+    det = np.zeros((np_, 3), np.float32)
+    src = np.zeros((np_, 3), np.float32)
+    fn(os.fsencode(path), ints.ctypes.data, floats.ctypes.data, det.ctypes.data, src.ctypes.data, np_, err, 512)
+    _, nx, ny, nz, nu, nv, nviews = (int(v) for v in ints)
+    return Geometry(nx, ny, nz, float(floats[0]), float(floats[1]), float(floats[2]), nu, nv, nviews,
+                    float(floats[3]), float(floats[4]), det, src)

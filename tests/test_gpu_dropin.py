@@ -71,3 +71,42 @@ def test_dropin_matches_oracle(tmp_path, oracle_mod):
     rc, saa = oracle_mod.saa(proj, gm.src[:, 1], planes, float(sid), fz, gm.sy)
     assert rc == 0
     np.testing.assert_array_equal(rd("saa.bin", (len(planes), gm.nv, gm.nu)), saa)
+
+
+def test_dropin_from_xml_matches_oracle(tmp_path, oracle_mod):
+    """The reference's calling sequence end to end: TomoGeometry read from an
+    XML geometry file (frame origin off the volume centre, fulcrum below
+    it), then Projector and recons:: on the GPU; bit-exact against the oracle
+    on the kernel arguments that file yields (from_xml, pinned against the
+    reference's own reading in tests/test_geometry_xml.py)."""
+    from kevernals_amd import geometry as G
+    from kevernals_amd.phantom import phantom
+
+    assert os.path.exists(CHECK), "drop-in not built"
+    g = G.scaled_down(G.small_plain(), 2)
+    fz = -12.0
+    G.to_xml(g, str(tmp_path / "geom.xml"), center=(10.5, -3.25, 7.0), fulcrum=(0.0, 0.0, fz))
+    gm = G.from_xml(str(tmp_path / "geom.xml"))
+    o = oracle_mod.Oracle(gm)
+    vol = phantom(gm, 5)
+    proj = o.fp_r(phantom(gm, 6))
+    vol.tofile(tmp_path / "vol.bin")
+    proj.tofile(tmp_path / "proj.bin")
+    r = subprocess.run([CHECK, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    status = dict(line.split() for line in open(tmp_path / "status.txt"))
+    assert status and all(v == "0" for v in status.values())
+
+    def rd(name, shape):
+        return np.fromfile(tmp_path / name, dtype=np.float32).reshape(shape)
+
+    np.testing.assert_array_equal(rd("fp.bin", gm.proj_shape), o.fp_r(vol))
+    np.testing.assert_array_equal(rd("bp.bin", gm.vol_shape), o.bp_r(proj))
+    np.testing.assert_array_equal(rd("mlem.bin", gm.vol_shape), o.mlem(proj, vol, 3, 1.0))
+    np.testing.assert_array_equal(rd("art.bin", gm.vol_shape), o.art(proj, vol, 2, 0.5))
+    planes = gm.plane_heights()
+    sid = np.float32(abs(np.float32(gm.det[0, 2]) - np.float32(gm.src[0, 2])))
+    rc, saa = oracle_mod.saa(proj, gm.src[:, 1], planes, float(sid), fz, gm.sy)
+    assert rc == 0
+    np.testing.assert_array_equal(rd("saa.bin", (len(planes), gm.nv, gm.nu)), saa)
