@@ -219,6 +219,33 @@ kv_status kv_cnn_gradient(kv_context *ctx, const float *d_params, float scale, c
 kv_status kv_cnn_adam(kv_context *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, int step,
                       float lr);
 
+/* ---- data path ------------------------------------------------------------
+ * PhantomMaker::GetPhantom (source/modules/dataHandling/PhantomMaker.cpp:9-51)
+ * on the context's volume grid: every voxel starts at `background`, then
+ * adds, in list order, the density of each box (Pave::Contains, Pave.h:51-59:
+ * inclusive bounds up to a relative FP32 epsilon) and then of each sphere
+ * (Sphere::Contains, Sphere.h:37-40: distance < radius) that contains the
+ * voxel's grid point.  Grid points are the voxels' lower corners, sequential
+ * FP32 sums of the spacing from the bottom-left-front corner
+ * (TomoVolume.cpp:193-229).  World mm, volume-centred frame.  Cylinders are
+ * stored but never applied by the reference (PhantomMaker.h:52 vs
+ * PhantomMaker.cpp:15-31) and have no counterpart here. */
+typedef struct kv_pave {
+    float x, y, z;     /* bottom-left-front corner */
+    float wx, wy, wz;  /* size */
+    float density;
+} kv_pave;
+typedef struct kv_sphere {
+    float cx, cy, cz, radius, density;
+} kv_sphere;
+kv_status kv_phantom(kv_context *ctx, float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,
+                     int nspheres, float *d_vol);
+/* vtkImageLogarithmicScale with constant c, as the reference's caller turns
+ * raw projection images into data (mainReconstruct.cpp:120-124, c = 1):
+ * out = c*log(1 + x) for x > 0, -c*log(1 - x) otherwise, evaluated in FP64
+ * and stored as FP32.  n values; d_out may be d_in. */
+kv_status kv_log_scale(kv_context *ctx, const float *d_in, float *d_out, int64_t n, double c);
+
 /* ---- trajectory extrapolation of mode-A MLEM ------------------------------
  * No counterpart in the reference: these serve the accelerated
  * reconstruction built on the mode-A MLEM update (a jump from iteration k to

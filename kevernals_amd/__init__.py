@@ -99,6 +99,8 @@ _SIGS = {
     "kv_cnn_gradient": (_I, [_P, _P, _F, _P, _P, _I, _I, _I, _P, ctypes.c_size_t, _P,
                              ctypes.POINTER(ctypes.c_double)]),
     "kv_cnn_adam": (_I, [_P, _P, _P, _P, _P, _I, _F]),
+    "kv_phantom": (_I, [_P, _F, _P, _I, _P, _I, _P]),
+    "kv_log_scale": (_I, [_P, _P, _P, ctypes.c_int64, ctypes.c_double]),
     "kv_traj_gram": (_I, [_P, _P, _I, _P]),
     "kv_traj_combine": (_I, [_P, _P, _I, _P, _P]),
 }
@@ -339,6 +341,21 @@ class Context:
 
     def cnn_adam(self, params, grad, m, v, step: int, lr: float = 1e-3):
         self._dev(self.lib.kv_cnn_adam, _ptr(params), _ptr(grad), _ptr(m), _ptr(v), step, lr)
+
+    def phantom(self, vol, background: float = 50.0, paves=(), spheres=()):
+        """PhantomMaker::GetPhantom on this geometry's voxel grid into vol:
+        paves are (x, y, z, wx, wy, wz, density) boxes (bottom-left-front
+        corner and size, mm), spheres (cx, cy, cz, radius, density).
+        Synchronous."""
+        pv = np.ascontiguousarray(np.asarray(paves, dtype=np.float32).reshape(-1, 7))
+        sp = np.ascontiguousarray(np.asarray(spheres, dtype=np.float32).reshape(-1, 5))
+        self._dev(self.lib.kv_phantom, float(background), _host_ptr(pv) if len(pv) else None, len(pv),
+                  _host_ptr(sp) if len(sp) else None, len(sp), _ptr(vol))
+
+    def log_scale(self, src, dst, c: float = 1.0):
+        """dst = c*log(1 + src) (sign-symmetric), the reference's data
+        transform (vtkImageLogarithmicScale, constant c)."""
+        self._dev(self.lib.kv_log_scale, _ptr(src), _ptr(dst), int(src.numel()), float(c))
 
     def traj_gram(self, iterates) -> np.ndarray:
         """Gram matrix (h x h, FP64) of the log-ratios of h+1 consecutive

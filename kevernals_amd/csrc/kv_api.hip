@@ -628,6 +628,69 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
     return KV_OK;
 }
 
+kv_status kv_phantom(kv_context *ctx, float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,
+                     int nspheres, float *d_vol)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_vol || npaves < 0 || nspheres < 0 || (npaves > 0 && !paves) || (nspheres > 0 && !spheres)) {
+        kv_set_error("bad phantom arguments");
+        return KV_ERR_INVALID;
+    }
+    const KvGeom &g = ctx->g;
+    // grid points: the bottom-left-front corner -0.5 * (n * spacing)
+    // (TomoGeometry.cpp:121-127, WSize3D.cpp:7-9) and sequential FP32 sums
+    // of the spacing from it (TomoVolume.cpp:193-215)
+    std::vector<float> pos((size_t)g.nx + g.ny + g.nz);
+    auto axis = [&](int n, float sp, float *dst) {
+        const float w = (float)n * sp;
+        dst[0] = (float)(-0.5 * (double)w);
+        for (int i = 1; i < n; ++i)
+            dst[i] = dst[i - 1] + sp;
+    };
+    axis(g.nx, g.sx, pos.data());
+    axis(g.ny, g.sy, pos.data() + g.nx);
+    axis(g.nz, g.sz, pos.data() + g.nx + g.ny);
+    const size_t bpos = pos.size() * sizeof(float), bpav = (size_t)npaves * sizeof(kv_pave),
+                 bsph = (size_t)nspheres * sizeof(kv_sphere);
+    const size_t opav = (bpos + 255) & ~(size_t)255, osph = (opav + bpav + 255) & ~(size_t)255;
+    char *d_buf = nullptr;
+    if (kv_status st = dev_alloc((void **)&d_buf, osph + bsph + 4))
+        return st;
+    hipError_t e = hipMemcpyAsync(d_buf, pos.data(), bpos, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && bpav)
+        e = hipMemcpyAsync(d_buf + opav, paves, bpav, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && bsph)
+        e = hipMemcpyAsync(d_buf + osph, spheres, bsph, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        const float *px = (const float *)d_buf;
+        e = kv_launch_phantom(g.nx, g.ny, g.nz, px, px + g.nx, px + g.nx + g.ny, background,
+                              (const kv_pave *)(d_buf + opav), npaves, (const kv_sphere *)(d_buf + osph), nspheres,
+                              d_vol, ctx->stream);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_buf);
+    if (e != hipSuccess) {
+        kv_set_error("phantom failed: %s", hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+kv_status kv_log_scale(kv_context *ctx, const float *d_in, float *d_out, int64_t n, double c)
+{
+    if (kv_status st = check_ctx(ctx, false))
+        return st;
+    if (!d_in || !d_out || n < 0 || !std::isfinite(c)) {
+        kv_set_error("bad log-scale arguments");
+        return KV_ERR_INVALID;
+    }
+    if (n > 0)
+        KV_TRY_HIP(kv_launch_log_scale(n, d_in, d_out, c, ctx->stream));
+    return KV_OK;
+}
+
 namespace {
 kv_status traj_args(kv_context *ctx, const float *const *d_iterates, int h, KvTraj &tr)
 {

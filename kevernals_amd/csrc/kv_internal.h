@@ -145,6 +145,14 @@ hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblock
 hipError_t kv_launch_loglik(const KvGeom &g, const float *data, const float *ax, double *partials, int nblocks,
                             hipStream_t s);
 
+// ---- data path (kv_misc.hip) --------------------------------------------
+// px/py/pz: the voxels' grid points per axis (device); paves/spheres: device
+// copies of the shape lists.
+hipError_t kv_launch_phantom(int nx, int ny, int nz, const float *px, const float *py, const float *pz,
+                             float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,
+                             int nspheres, float *out, hipStream_t s);
+hipError_t kv_launch_log_scale(int64_t n, const float *in, float *out, double c, hipStream_t s);
+
 // ---- trajectory extrapolation (kv_traj.hip) ------------------------------
 struct KvTraj {
     const float *x[KV_TRAJ_MAX + 1];  // iterates, oldest first

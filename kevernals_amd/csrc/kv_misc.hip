@@ -72,7 +72,116 @@ __global__ __launch_bounds__(256) void sum_kernel(int64_t n, const float *__rest
         partials[blockIdx.x] = red[0];
 }
 
+// AlmostEqualRelative(a, b) of commons/Maths.h:25-34 in FP32.
+__device__ __forceinline__ bool almost_equal_relative(float a, float b)
+{
+    const float diff = fabsf(a - b);
+    const float A = fabsf(a), B = fabsf(b);
+    const float m = (A < B) ? B : A;  // std::max
+    return diff <= __FLT_EPSILON__ * m;
+}
+
+// PhantomMaker::GetPhantom's per-voxel filler (PhantomMaker.cpp:13-29): the
+// background, then each containing box, then each containing sphere, added
+// in FP32 in list order.  The grid point of voxel (x, y, z) is
+// (px[x], py[y], pz[z]) (its lower corner, TomoVolume.cpp:193-229).  The
+// shape lists are staged in LDS by chunks.
+__global__ __launch_bounds__(256) void phantom_kernel(int nx, int ny, int nz, const float *__restrict__ px,
+                                                      const float *__restrict__ py, const float *__restrict__ pz,
+                                                      float background, const kv_pave *__restrict__ paves, int npaves,
+                                                      const kv_sphere *__restrict__ spheres, int nspheres,
+                                                      float *__restrict__ out)
+{
+    constexpr int kChunk = 256;
+    __shared__ kv_pave sp[kChunk];
+    __shared__ kv_sphere ss[kChunk];
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // every workgroup walks the same number of voxel rounds, so all of its
+    // threads reach every barrier
+    const int64_t rounds = (n + stride - 1) / stride;
+    for (int64_t r = 0; r < rounds; ++r) {
+        const int64_t i = first + r * stride;
+        const bool live = i < n;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        if (live) {
+            const int x = (int)(i % nx);
+            const int64_t q = i / nx;
+            const int y = (int)(q % ny), z = (int)(q / ny);
+            gx = px[x];
+            gy = py[y];
+            gz = pz[z];
+        }
+        float d = background;
+        for (int c0 = 0; c0 < npaves; c0 += kChunk) {
+            const int m = min(kChunk, npaves - c0);
+            __syncthreads();
+            if ((int)threadIdx.x < m)
+                sp[threadIdx.x] = paves[c0 + threadIdx.x];
+            __syncthreads();
+            for (int k = 0; k < m; ++k) {
+                const kv_pave P = sp[k];
+                // Pave::Contains (Pave.h:51-59)
+                const float ex = P.x + P.wx, ey = P.y + P.wy, ez = P.z + P.wz;
+                const bool in = (gx > P.x || almost_equal_relative(gx, P.x)) &&
+                                (gx < ex || almost_equal_relative(gx, ex)) &&
+                                (gy > P.y || almost_equal_relative(gy, P.y)) &&
+                                (gy < ey || almost_equal_relative(gy, ey)) &&
+                                (gz > P.z || almost_equal_relative(gz, P.z)) &&
+                                (gz < ez || almost_equal_relative(gz, ez));
+                d = in ? d + P.density : d;
+            }
+        }
+        for (int c0 = 0; c0 < nspheres; c0 += kChunk) {
+            const int m = min(kChunk, nspheres - c0);
+            __syncthreads();
+            if ((int)threadIdx.x < m)
+                ss[threadIdx.x] = spheres[c0 + threadIdx.x];
+            __syncthreads();
+            for (int k = 0; k < m; ++k) {
+                const kv_sphere S = ss[k];
+                // Sphere::Contains (Sphere.h:37-40): |centre - point| < radius,
+                // the norm as Vector3D::NormL2 (Vector3D.cpp:12-15)
+                const float dx = S.cx - gx, dy = S.cy - gy, dz = S.cz - gz;
+                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                d = dist < S.radius ? d + S.density : d;
+            }
+        }
+        if (live)
+            out[i] = d;
+    }
+}
+
+// vtkImageLogarithmicScale (VTK 9.0.1): c*log(1 + x) for x > 0,
+// -c*log(1 - x) otherwise, in FP64, stored as FP32.
+__global__ __launch_bounds__(256) void log_scale_kernel(int64_t n, const float *__restrict__ in,
+                                                        float *__restrict__ out, double c)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x = (double)in[i];
+        out[i] = (float)(x > 0.0 ? c * log(x + 1.0) : -c * log(1.0 - x));
+    }
+}
+
 }  // namespace
+
+hipError_t kv_launch_phantom(int nx, int ny, int nz, const float *px, const float *py, const float *pz,
+                             float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,
+                             int nspheres, float *out, hipStream_t s)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int nb = (int)std::min<int64_t>(16384, std::max<int64_t>(1, (n + 255) / 256));
+    phantom_kernel<<<nb, 256, 0, s>>>(nx, ny, nz, px, py, pz, background, paves, npaves, spheres, nspheres, out);
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_log_scale(int64_t n, const float *in, float *out, double c, hipStream_t s)
+{
+    const int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (n + 255) / 256));
+    log_scale_kernel<<<nb, 256, 0, s>>>(n, in, out, c);
+    return hipGetLastError();
+}
 
 hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s)
 {

@@ -54,3 +54,46 @@ extern "C" int ref_geometry_from_xml(const char *path, int32_t *ints, float *flo
     }
     return np;
 }
+
+#include "modules/geometry/Pave.h"
+#include "modules/geometry/Sphere.h"
+#include "modules/geometry/TomoVolume.h"
+
+// PhantomMaker::GetPhantom's densities (PhantomMaker.cpp:13-29) on the
+// reference's own TomoVolume grid, Pave::Contains and Sphere::Contains: the
+// volume is built as TomoGeometry.cpp:121-127 builds it from a size and a
+// spacing; paves are 7 floats (corner, size, density), spheres 5 (centre,
+// radius, density).  PhantomMaker.cpp itself needs VTK for its output image;
+// its loop is restated here around the reference's geometry classes, which
+// decide every bit.  Returns the voxel count.
+extern "C" int ref_phantom(int nx, int ny, int nz, float sx, float sy, float sz, float background,
+                           const float *paves, int npaves, const float *spheres, int nspheres, float *out)
+{
+    const VoxelSpacing spacing(sx, sy, sz);
+    const Size3D size(nx, ny, nz);
+    const WSize3D wsize(spacing, size);
+    const Position3D blf(-0.5 * wsize.x, -0.5 * wsize.y, -0.5 * wsize.z);
+    TomoVolume volume(blf, size, spacing);
+    const auto grid = volume.GetGridPositions();
+    std::vector<Pave> pv;
+    std::vector<Sphere> sp;
+    for (int k = 0; k < npaves; ++k) {
+        const float *q = paves + 7 * k;
+        pv.emplace_back(Position3D(q[0], q[1], q[2]), WSize3D(q[3], q[4], q[5]));
+    }
+    for (int k = 0; k < nspheres; ++k) {
+        const float *q = spheres + 5 * k;
+        sp.emplace_back(Position3D(q[0], q[1], q[2]), q[3]);
+    }
+    for (size_t i = 0; i < grid.size(); ++i) {
+        float d = background;
+        for (int k = 0; k < npaves; ++k)
+            if (pv[(size_t)k].Contains(grid[i]))
+                d += paves[7 * k + 6];
+        for (int k = 0; k < nspheres; ++k)
+            if (sp[(size_t)k].Contains(grid[i]))
+                d += spheres[5 * k + 4];
+        out[i] = d;
+    }
+    return (int)grid.size();
+}
