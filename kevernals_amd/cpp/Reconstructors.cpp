@@ -177,10 +177,18 @@ void Projector::PerformProjection(const vtkSmartPointer<vtkImageData> p_volume, 
     p_projectionsContainer->GetSpacing(psp);
     const std::vector<Position2D> bl = m_tomoGeometry->projectionsRoisBottomLeftPositions();
     const int views = pd[2] > 0 ? pd[2] : 1;
+    // Projector.cu:853-858: view p's detector origin is the geometry's region
+    // corner of view p (.at(p): a container with more views than the
+    // geometry is an error)
+    if ((size_t)views > bl.size()) {
+        lastError_ = make_error_code(ProjectorErrorCode::GeometryInconsistency);
+        lastMessage_ = "container has more views than the geometry";
+        return;
+    }
     std::vector<float> det((size_t)views * 3), src((size_t)views * 3);
     for (int p = 0; p < views; ++p) {
-        det[3 * p] = bl.empty() ? 0.f : bl[0].x;
-        det[3 * p + 1] = bl.empty() ? 0.f : bl[0].y;
+        det[3 * p] = bl[(size_t)p].x;
+        det[3 * p + 1] = bl[(size_t)p].y;
         det[3 * p + 2] = m_tomoGeometry->detectorsZCommonPosition();
         src[3 * p] = p_sourcePosition.x;
         src[3 * p + 1] = p_sourcePosition.y;

@@ -85,9 +85,10 @@ public:
     vtkSmartPointer<vtkImageData> PerformProjection(vtkSmartPointer<vtkImageData> p_volume) const;
     // Single-source projection into a caller container (unfinished in the
     // reference, Projector.cu:823-870): the volume is projected from
-    // p_sourcePosition onto the container, whose dimensions give the detector
-    // size and whose spacing gives the pixel pitch, detector origin at the
-    // first view's region bottom-left and the common detector height.
+    // p_sourcePosition onto the container's views, whose dimensions give the
+    // detector size and whose spacing gives the pixel pitch; view p's
+    // detector origin is the geometry's region corner of view p at the
+    // common detector height (Projector.cu:853-858).
     void PerformProjection(const vtkSmartPointer<vtkImageData> p_volume, const Position3D &p_sourcePosition,
                            vtkSmartPointer<vtkImageData> p_projectionsContainer) const;
     vtkSmartPointer<vtkImageData> PerformBackProjection(vtkSmartPointer<vtkImageData> p_projections) const;

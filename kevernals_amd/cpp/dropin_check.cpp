@@ -150,6 +150,18 @@ int main(int argc, char **argv)
     if (saa)
         write_raw(dir + "/saa.bin", saa.value());
 
+    // single-source projection (Projector.cu:823-870) from view 0's
+    // marshalled source position onto a container of every view
+    {
+        auto cont = image(nu, nv, np, pu, pv, 1.0);
+        const Position3D s0(geom.sourcesXCommonPosition(), geom.sourcesYPositions().at(0),
+                            geom.sourcesZCommonPosition());
+        projector.PerformProjection(vol, s0, cont);
+        note("single_source", projector.lastError());
+        if (!projector.lastError())
+            write_raw(dir + "/single.bin", cont);
+    }
+
     // error path: one reconstructed slice (Reconstructors.h:274-279)
     if (!from_xml) {
         TomoGeometry flat(Size3D(nx, ny, 1), VoxelSpacing(sx, sy, sz), Size2D(nu, nv), PixelSpacing(pu, pv), src, bl,

@@ -66,6 +66,10 @@ def test_dropin_matches_oracle(tmp_path, oracle_mod):
     np.testing.assert_array_equal(rd("mlem_default.bin", gm.vol_shape),
                                   o.mlem(proj, np.ones(gm.vol_shape, np.float32), 2, 1.0))
     np.testing.assert_array_equal(rd("art.bin", gm.vol_shape), o.art(proj, vol, 2, 0.5))
+    # single-source projection: every view from view 0's source position
+    g1 = G.Geometry(gm.nx, gm.ny, gm.nz, gm.sx, gm.sy, gm.sz, gm.nu, gm.nv, gm.nviews, gm.pu, gm.pv, gm.det,
+                    np.repeat(gm.src[:1], gm.nviews, axis=0))
+    np.testing.assert_array_equal(rd("single.bin", gm.proj_shape), oracle_mod.Oracle(g1).fp_r(vol))
     planes = gm.plane_heights()
     sid = np.float32(abs(np.float32(dz) - np.float32(sz)))
     rc, saa = oracle_mod.saa(proj, gm.src[:, 1], planes, float(sid), fz, gm.sy)
