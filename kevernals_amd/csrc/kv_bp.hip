@@ -599,7 +599,7 @@ __global__ __launch_bounds__(256, 5) void bp_tiled_kernel(KvGeom g, KvTables t, 
 // workgroups claim bricks from a global counter until the list is exhausted.
 // Mode A divides by each reconstruction's sensitivity A^T 1.
 template <int MODE>
-__global__ __launch_bounds__(256, MODE == KV_MODE_A ? 5 : 1) void bp_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+__global__ __launch_bounds__(256, 5) void bp_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
                                                        long long total, unsigned long long *counter, int iteration)
 {
     __shared__ BpShared<kZT> sm;
