@@ -361,12 +361,16 @@ def accelerated_mlem(ctx: Context, data, schedule=None, cnn: Optional[Cnn] = Non
             continue
         if note("mlem", k, x):
             break
-        if k == sch.first and k < sch.cost:
+        if k == sch.first and k < sch.cost and k < iters:
             # the probe: x becomes the jump to iteration `mid`
             jm.first_jump(cand)
             x.copy_(cand)
             jm.record(x)
             probing = True
+    if probing:
+        # stopped inside the probe: the estimate is still iteration `first`
+        x.copy_(jm.estimate)
+        note("mlem", k, x)
     run.x = x
     return run
 
@@ -452,11 +456,17 @@ def batched_accelerated_mlem(batch, schedule=None, params: Optional[np.ndarray] 
             note("final", k, False)
             continue
         note("probe" if probing else "mlem", k, probing)
-        if k == sch.first and k < sch.cost:
+        if k == sch.first and k < sch.cost and k < iters:
             for i in range(n):
                 jms[i].first_jump(cand[i])
                 store(i, cand[i])
                 jms[i].record(cand[i])
             torch.cuda.synchronize()
             probing = True
+    if probing:
+        # stopped inside the probe: every estimate is still iteration `first`
+        for i in range(n):
+            store(i, jms[i].estimate)
+        torch.cuda.synchronize()
+        note("mlem", k, False)
     return run

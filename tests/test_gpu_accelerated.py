@@ -134,3 +134,36 @@ def test_trained_network_gains_over_the_schedule(K):
         print(f"[accel] seed {seed}: schedule {r0:.5f}, schedule + CNN {r1:.5f} ({learned.trace[-1][0]})", flush=True)
         assert learned.trace[-1][0] == "cnn+"
         assert r1 < r0
+
+
+def test_runs_that_stop_early_return_the_estimate(K):
+    """iters below the schedule's cost: stopping before the probe leaves
+    plain MLEM; stopping inside the probe returns iteration `first` (the
+    held estimate), not the probe's state; batched likewise; more iterations
+    than the cost continue plain MLEM from the final estimate."""
+    from kevernals_amd import geometry as G
+    from kevernals_amd.learned import accelerated_mlem, batched_accelerated_mlem
+    from kevernals_amd.trajectory import Schedule
+
+    g = G.arc_geometry((64, 48, 12), (0.5, 0.5, 2.0), (128, 96), (0.5, 0.5), 7, 15.0)
+    ctx, data = problem(K, g, 8400)
+    sch = Schedule()
+    x = torch.ones(g.vol_shape, device="cuda")
+    plain = {}
+    for k in range(1, sch.first + 1):
+        ctx.mlem_iteration(x, data, 1.0, K.MODE_A)
+        plain[k] = x.clone()
+    for iters, want in ((6, plain[6]), (sch.first, plain[sch.first]), (sch.first + 3, plain[sch.first])):
+        run = accelerated_mlem(ctx, data, sch, iters=iters)
+        np.testing.assert_array_equal(run.x.cpu().numpy(), want.cpu().numpy(), err_msg=f"iters {iters}")
+        b = K.Batch([g], mode=K.MODE_A)
+        b.set_volume(0, np.ones(g.vol_shape, np.float32))
+        b.set_data(0, data.cpu().numpy())
+        batched_accelerated_mlem(b, sch, iters=iters)
+        np.testing.assert_array_equal(b.get_volume(0), want.cpu().numpy(), err_msg=f"batched iters {iters}")
+    full = accelerated_mlem(ctx, data, sch)
+    more = accelerated_mlem(ctx, data, sch, iters=sch.cost + 2)
+    y = full.x.clone()
+    for _ in range(2):
+        ctx.mlem_iteration(y, data, 1.0, K.MODE_A)
+    np.testing.assert_array_equal(more.x.cpu().numpy(), y.cpu().numpy())
