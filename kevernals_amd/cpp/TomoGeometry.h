@@ -33,6 +33,12 @@ struct Position3D {
     Position3D(float x_, float y_, float z_) : x(x_), y(y_), z(z_) {}
 };
 using WSize3D = Position3D;
+// A detector pixel index (geometry/Pixel.h).
+struct Pixel {
+    int x = 0, y = 0;
+    Pixel() = default;
+    Pixel(int x_, int y_) : x(x_), y(y_) {}
+};
 struct Size2D {
     int x = 0, y = 0;
     Size2D() = default;
@@ -145,6 +151,8 @@ public:
     Size2D projectionsRoisSize() const { return detSize_; }
     PixelSpacing projectionsPixelSpacing() const { return pixelSpacing_; }
     std::vector<Position2D> projectionsRoisBottomLeftPositions() const { return roiBL_; }
+    // bottom-left pixel of the region on the full detector (TomoGeometry.cpp:426)
+    Pixel GetProjectionROIsBLPixelPositionOnDetector() const { return roiPixel_; }
 
     // volume
     Size3D volumeSize3D() const { return volume_->GetSize3D(); }
@@ -158,6 +166,7 @@ private:
     Size2D detSize_;
     PixelSpacing pixelSpacing_;
     std::vector<Position2D> roiBL_;
+    Pixel roiPixel_;
     std::vector<float> sourcesY_;
     float srcX_ = 0.f, srcZ_ = 0.f, detZ_ = 0.f;
     WSize3D fulcrum_;
@@ -173,6 +182,7 @@ struct TomoGeometryDescription {
     Size3D volumeSize;
     VoxelSpacing voxelSpacing;
     Size2D roiSize;
+    Pixel roiPixel;
     PixelSpacing pixelSpacing;
     std::vector<Position3D> sources;
     std::vector<Position2D> roiBottomLefts;
@@ -188,6 +198,7 @@ inline TomoGeometry::TomoGeometry(const TomoGeometryDescription &d)
                    d.detectorZs, d.fulcrum)
 {
     valid_ = valid_ && d.ok;
+    roiPixel_ = d.roiPixel;
     error_ = d.ok ? (valid_ ? std::string() : std::string("geometry is empty")) : d.error;
 }
 

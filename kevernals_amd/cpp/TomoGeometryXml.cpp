@@ -43,6 +43,7 @@
 
 #include "Marshal.h"
 #include "TomoGeometry.h"
+#include "kevernals_dropin.h"
 #include "xml_lite.h"
 
 namespace {
@@ -273,6 +274,7 @@ struct Reader {
         if (tr[0] < bl[0] || tr[1] < bl[1])
             return fail("empty region of interest");
         d.roiSize = Size2D(tr[0] - bl[0] + 1, tr[1] - bl[1] + 1);
+        d.roiPixel = Pixel(bl[0], bl[1]);
         for (const Position2D &c : corners) {
             // pixel positions are FP32 running sums of the pitch
             float x = c.x, y = c.y;
@@ -339,4 +341,25 @@ extern "C" int kvd_geometry_from_xml(const char *path, int32_t *ints, float *flo
     if (err && err_len > 0)
         err[0] = '\0';
     return m.g.np;
+}
+
+extern "C" int kvd_geometry_roi_from_xml(const char *path, int32_t *roi, char *err, int err_len)
+{
+    if (!path || !roi) {
+        if (err && err_len > 0)
+            std::snprintf(err, (size_t)err_len, "null argument");
+        return 1;
+    }
+    const TomoGeometry tg{std::string(path)};
+    if (!tg.IsValid()) {
+        if (err && err_len > 0)
+            std::snprintf(err, (size_t)err_len, "%s", tg.ErrorMessage().c_str());
+        return 1;
+    }
+    const Pixel bl = tg.GetProjectionROIsBLPixelPositionOnDetector();
+    const Size2D size = tg.projectionsRoisSize();
+    roi[0] = bl.x, roi[1] = bl.y, roi[2] = size.x, roi[3] = size.y;
+    if (err && err_len > 0)
+        err[0] = '\0';
+    return 0;
 }
