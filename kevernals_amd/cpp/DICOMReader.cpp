@@ -274,15 +274,10 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
         Parser meta(bytes.data(), bytes.size(), Syntax{});
         meta.seek(start);
         std::string ts;
-        while (!meta.done()) {
-            const size_t at = meta.pos();
+        while (meta.pos() + 2 <= meta.size() && meta.u16(meta.pos()) == 0x0002) {
             Element e;
             if (!meta.header(&e))
                 return fail(meta.error());
-            if (e.group != 0x0002) {
-                meta.seek(at);
-                break;
-            }
             if (e.element == 0x0010)
                 ts = text(meta, e);
             if (!meta.skip(e))

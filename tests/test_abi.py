@@ -50,3 +50,14 @@ def test_no_cpu_fallback_in_product_package():
                 text = open(os.path.join(root, f), errors="ignore").read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "liboracle" not in text, f
+
+
+def test_dropin_library_exports_what_its_header_declares():
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(K.__file__)))
+    header = open(os.path.join(root, "include", "kevernals_dropin.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|void|const float|kvd_image)\s*\*?\s*(kvd_\w+)\s*\(", header, re.M))
+    lib = os.path.join(os.path.dirname(K.__file__), "lib", "libkevernals_dropin.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True)
+    exported = {ln.split()[-1] for ln in out.stdout.splitlines() if " T kvd_" in ln}
+    assert declared and exported == declared
