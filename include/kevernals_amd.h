@@ -119,6 +119,21 @@ kv_status kv_mlem_backward(kv_context *ctx, float *d_vol, int mode);
  * projector, x += BP(...) into the backprojector.  Mode R only. */
 kv_status kv_sart_iteration(kv_context *ctx, float *d_vol, const float *d_data, float relax);
 
+/* The elementwise steps of recons::ART and recons::MLEM on their own, for
+ * callers that need the intermediate images (the verbose TIFF dumps,
+ * Reconstructors.h:78-81,98-101,195-198,218-221).  kv_correct_projections
+ * turns a mode-R forward projection d_proj (kv_forward_project) into the
+ * corrected projection in place: relax*(data - FP) for KV_LOOP_ART
+ * (Reconstructors.h:87-93), relax*(data/FP) except where FP is zero for
+ * KV_LOOP_MLEM (:204-214).  kv_apply_update applies a mode-R backprojection
+ * d_bp (kv_back_project) to d_vol: x += b (ART, :113-121) or x *= b (MLEM,
+ * :235-241).  With the two projections these give the fused iterations'
+ * values bit for bit. */
+#define KV_LOOP_ART 0
+#define KV_LOOP_MLEM 1
+kv_status kv_correct_projections(kv_context *ctx, int loop, float relax, const float *d_data, float *d_proj);
+kv_status kv_apply_update(kv_context *ctx, int loop, float *d_vol, const float *d_bp);
+
 /* recons::ShiftAndAdd (Reconstructors.h:263-379).  src_y (host, np floats)
  * are the source y positions, plane_z (host, nplanes floats) the slice
  * heights, sid the source-to-imager distance, fulcrum_z the fulcrum height,

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "Marshal.h"
+#include "TiffWriter.h"
 #include "kevernals_amd.h"
 
 #ifdef KV_USE_REFERENCE_TREE
@@ -250,9 +251,27 @@ namespace recons {
 namespace {
 enum class Loop { Art, Mlem };
 
-// Iterations of recons::ART / recons::MLEM (Reconstructors.h:19-253).
+// A verbose-mode dump: the image written as TIFF to the output directory
+// path + name (the reference concatenates the two, Reconstructors.h:50).
+void dump(const std::optional<std::string> &dir, const std::string &name, vtkImageData *img)
+{
+    if (!dir.has_value())
+        return;
+    auto writer = vtkSmartPointer<vtkTIFFWriter>::New();
+    writer->SetFileName((dir.value() + name).c_str());
+    writer->SetInputData(img);
+    writer->Write();
+    if (writer->GetErrorCode())
+        std::cout << writer->GetErrorMessage() << std::endl;
+}
+
+// Iterations of recons::ART / recons::MLEM (Reconstructors.h:19-253).  With
+// an output directory the steps run apart (forward projection, correction,
+// backprojection, update) so that each intermediate image can be written
+// (Reconstructors.h:47-53,76-82,96-102,125-131 and :164-170,193-199,
+// 216-222,244-250); the values are those of the fused iterations.
 Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int iterations, float relax,
-                             std::optional<ImageDataPtr> initial)
+                             std::optional<ImageDataPtr> initial, const std::optional<std::string> &outdir)
 {
     // MLEM reports its failures with the ART code too (Reconstructors.h:159,190,230)
     const auto fail = ReconstructorsErrorCode::ART;
@@ -267,6 +286,7 @@ Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int
     }
     if (vol == nullptr || proj == nullptr || !tg)
         return fail;
+    dump(outdir, "initialVolume.tiff", vol);
     Projector projector{tg};
     kv_context *ctx = projector.context();
     if (!ctx)
@@ -288,11 +308,34 @@ Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int
             return fail;
         if (kv_memcpy_h2d(ctx, dv.p, x, nvox * 4) || kv_memcpy_h2d(ctx, dd.p, proj->GetScalarPointer(), nray * 4))
             return fail;
-        for (int it = 0; it < iterations; ++it) {
-            const kv_status st = kind == Loop::Art ? kv_sart_iteration(ctx, dv.f(), dd.f(), relax)
-                                                   : kv_mlem_iteration(ctx, dv.f(), dd.f(), relax, KV_MODE_R);
-            if (st != KV_OK)
+        if (!outdir.has_value()) {
+            for (int it = 0; it < iterations; ++it) {
+                const kv_status st = kind == Loop::Art ? kv_sart_iteration(ctx, dv.f(), dd.f(), relax)
+                                                       : kv_mlem_iteration(ctx, dv.f(), dd.f(), relax, KV_MODE_R);
+                if (st != KV_OK)
+                    return fail;
+            }
+        } else {
+            const int loop = kind == Loop::Art ? KV_LOOP_ART : KV_LOOP_MLEM;
+            const PixelSpacing ps = tg->projectionsPixelSpacing();
+            auto cur = new_image(ds.x, ds.y, np, ps.x, ps.y, 1.);  // Projector.cu:965-969
+            float *c = static_cast<float *>(cur->GetScalarPointer());
+            DevBuf dp, db;
+            if (!dp.alloc(ctx, nray * 4) || !db.alloc(ctx, nvox * 4))
                 return fail;
+            for (int it = 0; it < iterations; ++it) {
+                const std::string step = std::to_string(it) + ".tiff";
+                if (kv_forward_project(ctx, dv.f(), dp.f(), KV_MODE_R) || kv_memcpy_d2h(ctx, c, dp.p, nray * 4))
+                    return fail;
+                dump(outdir, "currentProjectionStep" + step, cur);
+                if (kv_correct_projections(ctx, loop, relax, dd.f(), dp.f()) || kv_memcpy_d2h(ctx, c, dp.p, nray * 4))
+                    return fail;
+                dump(outdir, "currentCorrectedProjectionStep" + step, cur);
+                if (kv_back_project(ctx, dp.f(), db.f(), KV_MODE_R) || kv_apply_update(ctx, loop, dv.f(), db.f()) ||
+                    kv_memcpy_d2h(ctx, x, dv.p, nvox * 4))
+                    return fail;
+                dump(outdir, "reconstructionstep" + step, vol);
+            }
         }
         if (kv_memcpy_d2h(ctx, x, dv.p, nvox * 4))
             return fail;
@@ -303,9 +346,11 @@ Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int
     // projection unchanged before backprojection, and a volume of another
     // shape skips the update (Reconstructors.h:87,115,204,235).
     for (int it = 0; it < iterations; ++it) {
+        const std::string step = std::to_string(it) + ".tiff";
         auto cur = projector.PerformProjection(vol);
         if (cur == nullptr)
             return fail;
+        dump(outdir, "currentProjectionStep" + step, cur);
         float *c = static_cast<float *>(cur->GetScalarPointer());
         if (data_ok) {
             const float *p = static_cast<const float *>(proj->GetScalarPointer());
@@ -319,6 +364,7 @@ Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int
                 }
             }
         }
+        dump(outdir, "currentCorrectedProjectionStep" + step, cur);
         auto back = projector.PerformBackProjection(cur);
         if (back == nullptr)
             return fail;
@@ -327,6 +373,7 @@ Result<ImageDataPtr> iterate(Loop kind, TomoGeometry *tg, ImageDataPtr proj, int
             for (size_t i = 0; i < nvox; ++i)
                 x[i] = kind == Loop::Art ? x[i] + b[i] : x[i] * b[i];
         }
+        dump(outdir, "reconstructionstep" + step, vol);
     }
     return vol;
 }
@@ -336,18 +383,16 @@ Result<ImageDataPtr> ART(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projection
                          float p_relaxationCoefficient, std::optional<ImageDataPtr> p_initialVolume,
                          std::optional<std::string> p_outputDirectoryPath)
 {
-    (void)p_outputDirectoryPath;
     return iterate(Loop::Art, p_tomoGeometry, p_projectionImages, p_iterationNumber, p_relaxationCoefficient,
-                   p_initialVolume);
+                   p_initialVolume, p_outputDirectoryPath);
 }
 
 Result<ImageDataPtr> MLEM(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectionImages, int p_iterationNumber,
                           float p_relaxationCoefficient, std::optional<ImageDataPtr> p_initialVolume,
                           std::optional<std::string> p_outputDirectoryPath)
 {
-    (void)p_outputDirectoryPath;
     return iterate(Loop::Mlem, p_tomoGeometry, p_projectionImages, p_iterationNumber, p_relaxationCoefficient,
-                   p_initialVolume);
+                   p_initialVolume, p_outputDirectoryPath);
 }
 
 Result<ImageDataPtr> BackProjection(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectionImages)
@@ -362,9 +407,12 @@ Result<ImageDataPtr> BackProjection(TomoGeometry *p_tomoGeometry, ImageDataPtr p
 Result<ImageDataPtr> ShiftAndAdd(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectionImages,
                                  std::optional<std::string> p_outputDirectoryPath)
 {
-    (void)p_outputDirectoryPath;
     if (!p_tomoGeometry || p_projectionImages == nullptr || !p_projectionImages->GetScalarPointer())
         return ReconstructorsErrorCode::ShiftAndAdd;
+    if (p_outputDirectoryPath.has_value()) {  // Reconstructors.h:288-298
+        std::cout << "zF = " << std::to_string(p_tomoGeometry->fulcrum().z) << std::endl;
+        std::cout << "sid = " << std::to_string(p_tomoGeometry->sid()) << std::endl;
+    }
     Projector projector{p_tomoGeometry};
     kv_context *ctx = projector.context();
     if (!ctx)

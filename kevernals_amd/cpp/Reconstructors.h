@@ -10,7 +10,9 @@
 //    (the reference called exit() from helper_cuda);
 //  * ART/MLEM keep volume, data and work buffers on the device for all
 //    iterations (the reference copied both ways every projection);
-//  * the verbose TIFF dumps of p_outputDirectoryPath are not written.
+//  * with p_outputDirectoryPath, ART and MLEM run their steps apart on the
+//    device to write the reference's TIFF dumps (same values as the fused
+//    iterations).
 #pragma once
 
 //

@@ -8,9 +8,12 @@
 //   <dir>/geom.txt : nx ny nz sx sy sz nu nv np pu pv fulcrum_z, then per view
 //                    src_x src_y src_z det_x det_y det_z
 //   <dir>/vol.bin, <dir>/proj.bin : float32 volume and projections
-// Writes fp.bin, bp.bin, mlem.bin, mlem_default.bin, art.bin, saa.bin and
+// Writes fp.bin, bp.bin, mlem.bin, mlem_default.bin, art.bin, saa.bin,
+// mlem_verbose.bin and art_verbose.bin (with TIFF dumps under dump_mlem/ and
+// dump_art/) and
 // status.txt (one "name code" line per call).
 #include <cstdio>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <memory>
@@ -143,6 +146,20 @@ int main(int argc, char **argv)
     note("art", art.has_value() ? std::error_code() : art.error());
     if (art)
         write_raw(dir + "/art.bin", art.value());
+    // verbose mode: the TIFF dumps of every step (Reconstructors.h:47-131,164-250)
+    {
+        const std::string dm = dir + "/dump_mlem/", da = dir + "/dump_art/";
+        std::filesystem::create_directories(dm);
+        std::filesystem::create_directories(da);
+        auto vm = recons::MLEM(&geom, proj, 3, 1.0f, copy(vol), dm);
+        note("mlem_verbose", vm.has_value() ? std::error_code() : vm.error());
+        if (vm)
+            write_raw(dir + "/mlem_verbose.bin", vm.value());
+        auto va = recons::ART(&geom, proj, 2, 0.5f, copy(vol), da);
+        note("art_verbose", va.has_value() ? std::error_code() : va.error());
+        if (va)
+            write_raw(dir + "/art_verbose.bin", va.value());
+    }
     auto bpr = recons::BackProjection(&geom, proj);
     note("recons_backprojection", bpr.has_value() ? std::error_code() : bpr.error());
     auto saa = recons::ShiftAndAdd(&geom, proj, std::nullopt);

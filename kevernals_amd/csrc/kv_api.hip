@@ -523,6 +523,30 @@ kv_status kv_sart_iteration(kv_context *ctx, float *d_vol, const float *d_data, 
     return KV_OK;
 }
 
+kv_status kv_correct_projections(kv_context *ctx, int loop, float relax, const float *d_data, float *d_proj)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_data || !d_proj || (loop != KV_LOOP_ART && loop != KV_LOOP_MLEM)) {
+        kv_set_error("bad correction arguments");
+        return KV_ERR_INVALID;
+    }
+    KV_TRY_HIP(kv_launch_correct(kv_nray(ctx->g), loop, relax, d_data, d_proj, ctx->stream));
+    return KV_OK;
+}
+
+kv_status kv_apply_update(kv_context *ctx, int loop, float *d_vol, const float *d_bp)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_vol || !d_bp || (loop != KV_LOOP_ART && loop != KV_LOOP_MLEM)) {
+        kv_set_error("bad update arguments");
+        return KV_ERR_INVALID;
+    }
+    KV_TRY_HIP(kv_launch_update(kv_nvox(ctx->g), loop, d_vol, d_bp, ctx->stream));
+    return KV_OK;
+}
+
 kv_status kv_sensitivity(kv_context *ctx, float *d_vol)
 {
     if (kv_status st = check_ctx(ctx, true))

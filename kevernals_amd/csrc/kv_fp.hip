@@ -247,14 +247,6 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
 }
 
 // AlmostEqualRelative(v, 0) of commons/Maths.h:25-34 in FP32.
-__device__ __forceinline__ bool almost_equal_zero(float a)
-{
-    const float diff = fabsf(a - 0.f);
-    const float A = fabsf(a), B = 0.f;
-    const float m = (A < B) ? B : A;  // std::max
-    return diff <= __FLT_EPSILON__ * m;
-}
-
 // One ray of the mode-R forward pass with its epilogue.
 template <int EP>
 __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restrict__ vol, float *__restrict__ out,

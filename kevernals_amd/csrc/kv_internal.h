@@ -106,6 +106,16 @@ kv_status kv_internal_ensure_sens(kv_context *ctx);
         }                                                                                       \
     } while (0)
 
+// AlmostEqualRelative(a, 0.F) of commons/Maths.h:25-34: the projection
+// values MLEM leaves unchanged (Reconstructors.h:207-213).
+__device__ __forceinline__ bool almost_equal_zero(float a)
+{
+    const float diff = fabsf(a - 0.f);
+    const float A = fabsf(a), B = 0.f;
+    const float m = (A < B) ? B : A;  // std::max
+    return diff <= __FLT_EPSILON__ * m;
+}
+
 inline int64_t kv_nvox(const KvGeom &g) { return (int64_t)g.nx * g.ny * g.nz; }
 inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 
@@ -151,6 +161,8 @@ hipError_t kv_launch_loglik(const KvGeom &g, const float *data, const float *ax,
 hipError_t kv_launch_phantom(int nx, int ny, int nz, const float *px, const float *py, const float *pz,
                              float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,
                              int nspheres, float *out, hipStream_t s);
+hipError_t kv_launch_correct(int64_t n, int loop, float relax, const float *data, float *proj, hipStream_t s);
+hipError_t kv_launch_update(int64_t n, int loop, float *vol, const float *bp, hipStream_t s);
 hipError_t kv_launch_log_scale(int64_t n, const float *in, float *out, double c, hipStream_t s);
 
 // ---- trajectory extrapolation (kv_traj.hip) ------------------------------

@@ -164,7 +164,44 @@ __global__ __launch_bounds__(256) void log_scale_kernel(int64_t n, const float *
     }
 }
 
+// Step 2 of recons::ART (Reconstructors.h:87-93) or recons::MLEM (:204-214)
+// on a forward projection: the forward projector's fused epilogues taken
+// apart.
+__global__ __launch_bounds__(256) void correct_kernel(int64_t n, int loop, float relax,
+                                                      const float *__restrict__ data, float *__restrict__ proj)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float fp = proj[i];
+        if (loop == KV_LOOP_MLEM)
+            proj[i] = almost_equal_zero(fp) ? fp : relax * (data[i] / fp);
+        else
+            proj[i] = relax * (data[i] - fp);
+    }
+}
+
+// Step 4: x += b (ART, Reconstructors.h:113-121) or x *= b (MLEM, :235-241).
+__global__ __launch_bounds__(256) void update_kernel(int64_t n, int loop, float *__restrict__ vol,
+                                                     const float *__restrict__ bp)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        vol[i] = loop == KV_LOOP_MLEM ? vol[i] * bp[i] : vol[i] + bp[i];
+}
+
 }  // namespace
+
+hipError_t kv_launch_correct(int64_t n, int loop, float relax, const float *data, float *proj, hipStream_t s)
+{
+    const int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (n + 255) / 256));
+    correct_kernel<<<nb, 256, 0, s>>>(n, loop, relax, data, proj);
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_update(int64_t n, int loop, float *vol, const float *bp, hipStream_t s)
+{
+    const int nb = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (n + 255) / 256));
+    update_kernel<<<nb, 256, 0, s>>>(n, loop, vol, bp);
+    return hipGetLastError();
+}
 
 hipError_t kv_launch_phantom(int nx, int ny, int nz, const float *px, const float *py, const float *pz,
                              float background, const kv_pave *paves, int npaves, const kv_sphere *spheres,

@@ -32,6 +32,39 @@ def marshalled(g):
     return Geometry(g.nx, g.ny, g.nz, g.sx, g.sy, g.sz, g.nu, g.nv, g.nviews, g.pu, g.pv, det, src), sz, dz
 
 
+def check_dumps(tmp_path, gm, o, vol, proj):
+    """Verbose ART/MLEM (steps run apart, TIFF dumps of each): the same final
+    volumes as the fused iterations, and every dumped image equal to the
+    oracle's value at that step (pages decoded by the TIFF oracle's reader)."""
+    from oracle import tiff as T
+
+    def rd(name, shape):
+        return np.fromfile(tmp_path / name, dtype=np.float32).reshape(shape)
+
+    def tif(path):
+        pages = T.read_tiff(open(path, "rb").read())
+        return np.stack([p[..., 0] for _, p in pages])
+
+    np.testing.assert_array_equal(rd("mlem_verbose.bin", gm.vol_shape), rd("mlem.bin", gm.vol_shape))
+    np.testing.assert_array_equal(rd("art_verbose.bin", gm.vol_shape), rd("art.bin", gm.vol_shape))
+    for which, iters, relax in (("mlem", 3, 1.0), ("art", 2, 0.5)):
+        d = tmp_path / f"dump_{which}"
+        np.testing.assert_array_equal(tif(d / "initialVolume.tiff"), vol)
+        run = o.mlem if which == "mlem" else o.art
+        x = vol
+        for it in range(iters):
+            fp = o.fp_r(x)
+            np.testing.assert_array_equal(tif(d / f"currentProjectionStep{it}.tiff"), fp)
+            r = np.float32(relax)
+            if which == "mlem":
+                corr = np.where(fp == 0, fp, r * (proj / np.where(fp == 0, 1, fp))).astype(np.float32)
+            else:
+                corr = (r * (proj - fp)).astype(np.float32)
+            np.testing.assert_array_equal(tif(d / f"currentCorrectedProjectionStep{it}.tiff"), corr)
+            x = run(proj, vol, it + 1, relax)
+            np.testing.assert_array_equal(tif(d / f"reconstructionstep{it}.tiff"), x)
+
+
 def test_dropin_matches_oracle(tmp_path, oracle_mod):
     from kevernals_amd import geometry as G
     from kevernals_amd.phantom import phantom
@@ -66,6 +99,7 @@ def test_dropin_matches_oracle(tmp_path, oracle_mod):
     np.testing.assert_array_equal(rd("mlem_default.bin", gm.vol_shape),
                                   o.mlem(proj, np.ones(gm.vol_shape, np.float32), 2, 1.0))
     np.testing.assert_array_equal(rd("art.bin", gm.vol_shape), o.art(proj, vol, 2, 0.5))
+    check_dumps(tmp_path, gm, o, vol, proj)
     # single-source projection: every view from view 0's source position
     g1 = G.Geometry(gm.nx, gm.ny, gm.nz, gm.sx, gm.sy, gm.sz, gm.nu, gm.nv, gm.nviews, gm.pu, gm.pv, gm.det,
                     np.repeat(gm.src[:1], gm.nviews, axis=0))
