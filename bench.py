@@ -235,6 +235,111 @@ def bench_cnn(args, torch, K, G):
     print(json.dumps(out), flush=True)
 
 
+def _dicom_frame_file(path, frame, acquisition_time):
+    """A minimal DICOM file (PS3.10, explicit VR little endian) holding one
+    16-bit frame: the input of the ingest measurement."""
+    import struct
+
+    def el(g, e, vr, v):
+        if len(v) % 2:
+            v += b"\x00" if vr in ("UI", "OB") else b" "
+        if vr in ("OB", "OW"):
+            return struct.pack("<HH", g, e) + vr.encode() + b"\x00\x00" + struct.pack("<I", len(v)) + v
+        return struct.pack("<HH", g, e) + vr.encode() + struct.pack("<H", len(v)) + v
+
+    meta = el(2, 1, "OB", b"\x00\x01") + el(2, 0x10, "UI", b"1.2.840.10008.1.2.1")
+    us = lambda e, v: el(0x28, e, "US", struct.pack("<H", v))
+    body = (el(8, 0x32, "TM", acquisition_time.encode()) + el(0x10, 0x20, "LO", b"BENCH") + us(2, 1) +
+            el(0x28, 4, "CS", b"MONOCHROME2") + us(0x10, frame.shape[0]) + us(0x11, frame.shape[1]) +
+            us(0x100, 16) + us(0x101, 14) + us(0x102, 13) + us(0x103, 0) +
+            el(0x7FE0, 0x10, "OW", np.ascontiguousarray(frame, "<u2").tobytes()))
+    with open(path, "wb") as f:
+        f.write(b"\x00" * 128 + b"DICM" + el(2, 0, "UL", struct.pack("<I", len(meta))) + meta + body)
+
+
+def bench_io(args, torch, K, G):
+    """The data formats either side of the path at the full size, on the
+    host: the TIFF dump of a 1024x1024x64 volume as recons:: verbose mode
+    writes it (device-to-host copy, PackBits, file write), and the DICOM
+    ingest of 25 projections of 2048x2048 16-bit pixels (read and decode,
+    crop to the detector region, host-to-device copy, log transform on the
+    GPU).  Random values: PackBits finds no runs (its worst case)."""
+    import shutil
+    import tempfile
+
+    from kevernals_amd import io as KIO
+
+    g = G.full_plain()
+    ctx = K.Context(0, g)
+    ctx.use_torch_stream()
+    tmp = tempfile.mkdtemp(prefix="kv_io_")
+    try:
+        vol = torch.rand(g.vol_shape, device="cuda")
+        host = torch.empty(g.vol_shape, dtype=torch.float32, pin_memory=True)
+        tif = os.path.join(tmp, "reconstructionstep0.tiff")
+        dump = []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            host.copy_(vol)
+            KIO.write_tiff(tif, host.numpy(), spacing=(g.sx, g.sy))
+            if i >= args.warmup:
+                dump.append(time.perf_counter() - t0)
+        vbytes = vol.numel() * 4
+        fsize = os.path.getsize(tif)
+        del vol, host
+        log(f"[bench] TIFF dump of {g.vol_shape}: {np.median(dump) * 1e3:.1f} ms, "
+            f"{vbytes / np.median(dump) / 1e9:.2f} GB/s, file {fsize / vbytes:.4f} x the volume")
+
+        rng = np.random.default_rng(3)
+        ddir = os.path.join(tmp, "dcm")
+        os.makedirs(ddir)
+        for p in range(g.nviews):
+            _dicom_frame_file(os.path.join(ddir, f"p{p:02d}.dcm"), rng.integers(0, 1 << 14, (g.nv, g.nu)),
+                              f"{120000 + p:06d}")
+        roi = (0, 0, g.nu, g.nv)
+        d = torch.empty((g.nviews, g.nv, g.nu), device="cuda")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ingest, log_ms = [], []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            img = KIO.read_dicom_directory(ddir, roi=roi)
+            d.copy_(torch.from_numpy(img.data))
+            ev[0].record()
+            ctx.log_scale(d, d, 1.0)
+            ev[1].record()
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                ingest.append(time.perf_counter() - t0)
+                log_ms.append(ev[0].elapsed_time(ev[1]))
+        pbytes = g.nviews * g.nv * g.nu * 2
+        t_ingest = float(np.median(ingest))
+        t_log = float(np.median(log_ms)) * 1e-3
+        log_gbs = d.numel() * 8 / t_log / 1e9
+        log(f"[bench] DICOM ingest of {g.nviews} x {g.nv}x{g.nu} u16: {t_ingest * 1e3:.1f} ms, "
+            f"{pbytes / t_ingest / 1e9:.2f} GB/s of pixel data; log transform {t_log * 1e3:.3f} ms "
+            f"({log_gbs:.0f} GB/s)")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {
+        "metric": METRIC, "value": round(pbytes / t_ingest / 1e9, 3), "unit": "GB/s (DICOM ingest, host)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_ingest * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16 in, f32 out",
+        "data": "synthetic (uniform random pixels and voxels)",
+        "config": {"workload": "DICOM ingest of 25 x 2048x2048 u16 projections; TIFF dump of a 1024x1024x64 "
+                               "float volume", "parallelism": "single GPU, host I/O"},
+        "io": {"tiff_dump_ms": round(float(np.median(dump)) * 1e3, 1),
+               "tiff_dump_gbs": round(vbytes / float(np.median(dump)) / 1e9, 3),
+               "tiff_size_ratio": round(fsize / vbytes, 5),
+               "dicom_ingest_ms": round(t_ingest * 1e3, 1), "log_transform_ms": round(t_log * 1e3, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(log_gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(log_gbs / 8000.0, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _times(trace, thr, rmse_of):
     """(seconds, MLEM iterations run) at the first trace entry with RMSE <= thr."""
     for e in trace:
@@ -447,7 +552,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "learned-batched", "cnn"],
+    ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "learned-batched", "cnn", "io"],
                     default="full")
     ap.add_argument("--recons", type=int, default=8, help="reconstructions of the learned-batched workload")
     ap.add_argument("--tile-cache", default=None, help="autotuner cache file (batched workload)")
@@ -472,6 +577,8 @@ def main():
         return bench_cnn(args, torch, K, G)
     if args.workload == "learned-batched":
         return bench_learned_batched(args, torch, K, G)
+    if args.workload == "io":
+        return bench_io(args, torch, K, G)
     g = G.full_plain() if args.workload == "full" else G.small_plain()
     ctx = K.Context(0, g)
     ctx.use_torch_stream()

@@ -2,6 +2,7 @@
 // a DICOM parser of its own: PS3.5 data encoding, PS3.10 file format).
 #include "DICOMReader.h"
 
+#include <omp.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -384,6 +385,7 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
     const int shift = high + 1 - stored;
     const uint64_t mask = stored == 32 ? 0xFFFFFFFFull : ((1ull << stored) - 1);
     const bool big = syntax.bigEndian;
+#pragma omp parallel for if (count >= (1u << 20) && !omp_in_parallel()) schedule(static)
     for (size_t k = 0; k < count; ++k) {
         uint32_t raw;
         const uint8_t *q = px + k * bpp;
@@ -482,20 +484,26 @@ bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indices
     std::sort(paths.begin(), paths.end());
     struct Entry {
         Frames frames;
-        std::string time, path;
+        std::string time, path, error;
+        bool ok = false;
     };
-    std::vector<Entry> images;
-    for (const std::string &path : paths) {
-        if (!CanReadFile(path))
-            return fail("Cannot read file: " + path);
-        Entry e;
-        if (!ReadFile(path, &e.frames, err))
-            return false;
-        if (e.frames.hasPatientID)  // DICOMReader.cpp:148-152
+    // files decode independently
+    std::vector<Entry> images(paths.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < (int)paths.size(); ++i) {
+        Entry &e = images[i];
+        e.path = paths[i];
+        if (!CanReadFile(e.path)) {
+            e.error = "Cannot read file: " + e.path;
+            continue;
+        }
+        e.ok = ReadFile(e.path, &e.frames, &e.error);
+        if (e.ok && e.frames.hasPatientID)  // DICOMReader.cpp:148-152
             e.time = e.frames.acquisitionTime;
-        e.path = path;
-        images.push_back(std::move(e));
     }
+    for (const Entry &e : images)
+        if (!e.ok)
+            return fail(e.error);
     std::stable_sort(images.begin(), images.end(), [](const Entry &a, const Entry &b) { return a.time < b.time; });
     std::vector<int> drop = indicesToRemove;
     std::sort(drop.begin(), drop.end(), std::greater<int>());
@@ -506,33 +514,39 @@ bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indices
     }
     if (images.empty())
         return fail("no DICOM images in " + dir);
-    std::vector<ImageDataPtr> crops;
-    int total = 0;
-    for (const Entry &e : images) {
+    // vtkExtractVOI of each image (clipped to it), appended along z
+    // (vtkImageAppend): every crop must cover the same x and y extent
+    std::vector<int> zoff(images.size() + 1, 0);
+    int ext[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < images.size(); ++i) {
+        const Frames &f = images[i].frames;
+        const int e[4] = {std::max(x0, 0), std::min(x0 + w - 1, f.columns - 1), std::max(y0, 0),
+                          std::min(y0 + h - 1, f.rows - 1)};
+        if (w <= 0 || h <= 0 || e[0] > e[1] || e[2] > e[3])
+            return fail("Empty data for raw projections " + images[i].path +
+                        ": the detector region lies outside the image");
+        if (i == 0)
+            std::copy(e, e + 4, ext);
+        else if (!std::equal(e, e + 4, ext))
+            return fail("image " + images[i].path + " differs in size from the first image");
         if (verbose)
-            std::cout << "Ordered raw projections " << e.path << std::endl;
-        std::string m;
-        auto c = CropToImage(e.frames, x0, y0, w, h, &m);
-        if (c == nullptr)
-            return fail("Empty data for raw projections " + e.path + ": " + m);
-        const int *ex = c->GetExtent(), *e0 = crops.empty() ? ex : crops.front()->GetExtent();
-        if (ex[0] != e0[0] || ex[1] != e0[1] || ex[2] != e0[2] || ex[3] != e0[3])
-            return fail("image " + e.path + " differs in size from the first image");
-        total += c->GetDimensions()[2];
-        crops.push_back(c);
+            std::cout << "Ordered raw projections " << images[i].path << std::endl;
+        zoff[i + 1] = zoff[i] + f.frames;
     }
-    // vtkImageAppend along z
-    const int *e0 = crops.front()->GetExtent();
-    const double *sp = crops.front()->GetSpacing();
     auto img = ImageDataPtr::New();
-    img->SetExtent(e0[0], e0[1], e0[2], e0[3], 0, total - 1);
-    img->SetSpacing(sp[0], sp[1], sp[2]);
+    img->SetExtent(ext[0], ext[1], ext[2], ext[3], 0, zoff.back() - 1);
+    img->SetSpacing(images[0].frames.spacing[0], images[0].frames.spacing[1], 1.0);
     img->AllocateScalars(VTK_FLOAT, 1);
     float *dst = static_cast<float *>(img->GetScalarPointer());
-    for (const auto &c : crops) {
-        const size_t n = c->GetNumberOfPoints();
-        const float *src = static_cast<const float *>(c->GetScalarPointer());
-        dst = std::copy(src, src + n, dst);
+    const size_t nx = (size_t)(ext[1] - ext[0] + 1), ny = (size_t)(ext[3] - ext[2] + 1);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < (int)images.size(); ++i) {
+        const Frames &f = images[i].frames;
+        for (int z = 0; z < f.frames; ++z)
+            for (int y = ext[2]; y <= ext[3]; ++y) {
+                const float *src = f.values.data() + ((size_t)z * f.rows + (f.rows - 1 - y)) * f.columns + ext[0];
+                std::copy(src, src + nx, dst + (((size_t)zoff[i] + z) * ny + (size_t)(y - ext[2])) * nx);
+            }
     }
     *out = img;
     return true;

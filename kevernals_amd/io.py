@@ -67,6 +67,14 @@ class Image:
         self.data, self.extent, self.spacing = data, tuple(extent), tuple(spacing)
 
 
+class _Owner:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        _lib().kvd_image_free(self.handle)
+
+
 def _read(path, directory, remove, roi):
     lib = _lib()
     err = ctypes.create_string_buffer(1024)
@@ -78,16 +86,15 @@ def _read(path, directory, remove, roi):
                            r.ctypes.data if r is not None else None, err, 1024)
     if not h:
         raise ValueError(err.value.decode(errors="replace"))
-    try:
-        ext = np.zeros(6, np.int32)
-        sp = np.zeros(3, np.float64)
-        lib.kvd_image_info(h, ext.ctypes.data, sp.ctypes.data)
-        shape = (ext[5] - ext[4] + 1, ext[3] - ext[2] + 1, ext[1] - ext[0] + 1)
-        n = int(np.prod(shape))
-        buf = (ctypes.c_float * n).from_address(lib.kvd_image_data(h))
-        data = np.frombuffer(buf, np.float32).reshape(shape).copy()
-    finally:
-        lib.kvd_image_free(h)
+    owner = _Owner(h)  # frees the image when the last view of it goes
+    ext = np.zeros(6, np.int32)
+    sp = np.zeros(3, np.float64)
+    lib.kvd_image_info(h, ext.ctypes.data, sp.ctypes.data)
+    shape = (ext[5] - ext[4] + 1, ext[3] - ext[2] + 1, ext[1] - ext[0] + 1)
+    n = int(np.prod(shape))
+    buf = (ctypes.c_float * n).from_address(lib.kvd_image_data(h))
+    buf._owner = owner
+    data = np.frombuffer(buf, np.float32).reshape(shape)
     return Image(data, ext.tolist(), sp.tolist())
 
 
