@@ -262,7 +262,8 @@ void dump(const std::optional<std::string> &dir, const std::string &name, vtkIma
     writer->SetInputData(img);
     writer->Write();
     if (writer->GetErrorCode())
-        std::cout << writer->GetErrorMessage() << std::endl;
+        std::cout << "TIFF dump " << dir.value() + name << " failed (error " << writer->GetErrorCode() << ")"
+                  << std::endl;
 }
 
 // Iterations of recons::ART / recons::MLEM (Reconstructors.h:19-253).  With

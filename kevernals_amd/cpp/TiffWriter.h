@@ -39,7 +39,9 @@ bool WriteImage(const std::string &path, const void *data, int scalar_type, int 
 
 }  // namespace kvtiff
 
-#ifndef KV_USE_REAL_VTK
+#ifdef KV_USE_REAL_VTK
+#include <vtkTIFFWriter.h>
+#else
 // The part of vtkTIFFWriter the reference calls: New, SetFileName,
 // SetInputData, Write (and the compression setters and error code).
 class vtkTIFFWriter {
