@@ -211,7 +211,8 @@ kv_status kv_batch_autotune(kv_batch *b, const char *cache_path);
  * accumulation on the matrix cores.  Parameters are one FP32 blob of
  * KV_CNN_NPARAMS values: W1[16][1][27] b1[16] W2[16][16][27] b2[16]
  * W3[16][16][27] b3[16] W4[1][16][27] b4[1], W[o][i][t] with tap
- * t = 9(dz+1) + 3(dy+1) + (dx+1).  Volumes float[nz][ny][nx].  The caller
+ * t = 9(dz+1) + 3(dy+1) + (dx+1).  Volumes float[nz][ny][nx], at most 2^30
+ * voxels and 22369621 voxels per slice (KV_ERR_INVALID beyond).  The caller
  * provides a 256-byte aligned device workspace of kv_cnn_workspace_size
  * bytes.  Asynchronous on the context's stream unless stated. */
 #define KV_CNN_NPARAMS 14737

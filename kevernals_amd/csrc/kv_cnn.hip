@@ -110,11 +110,14 @@ __device__ __forceinline__ void store4(const f32x4 &acc, int g, size_t v, const 
 }
 
 // Weight operands of every convolution, packed once per call as the MFMA A
-// operands (bf16): variant v, MFMA m, lane l at wpk[(v * 14 + m) * 64 + l].
-// conv16 variants use m = 0..13 (k-group g = l >> 4 holds tap 2m + (g >> 1),
-// channels 8(g & 1) .. +7); conv1 variants use m = 0 (k-group g holds taps
+// operands (bf16): variant v, operand m, lane l at wpk[(v * WV_OPS + m) * 64 + l].
+// conv16 variants: m = 2 (3 dz + dy) + p for the taps (dz, dy, dx) of the
+// pair p: p = 0 holds dx = 0 in k-groups 0-1 and dx = 1 in k-groups 2-3,
+// p = 1 holds dx = 2 in k-groups 0-1 and zeros in 2-3; k-group g covers
+// channels 8 (g & 1) .. +7.  conv1 variants use m = 0 (k-group g holds taps
 // 8g .. 8g+7).
 enum { WV_L2 = 0, WV_L3 = 1, WV_L4 = 2, WV_L3T = 3, WV_L2T = 4, WV_L1 = 5, WV_L4T = 6, WV_COUNT = 7 };
+constexpr int WV_OPS = 18;
 
 __global__ void pack_weights_kernel(const float *__restrict__ p, u32x4 *__restrict__ wpk, int ow1, int ow2, int ow3,
                                     int ow4)
@@ -131,7 +134,8 @@ __global__ void pack_weights_kernel(const float *__restrict__ p, u32x4 *__restri
     case WV_L1: w = p + ow1; cout = 16; cin = 1; tr = 0; break;
     default: w = p + ow4; cout = 1; cin = 16; tr = 1; break;
     }
-    for (int m = 0; m < 14; ++m) {
+    for (int m = 0; m < WV_OPS; ++m) {
+        const int pair = m & 1, dzdy = m >> 1, dx = pair ? 2 : (g >> 1);
         unsigned int e[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -139,16 +143,26 @@ __global__ void pack_weights_kernel(const float *__restrict__ p, u32x4 *__restri
             if (v >= WV_L1)
                 val = m == 0 ? w_elem(w, cout, cin, tr, col, 0, 8 * g + j) : 0.f;
             else
-                val = w_elem(w, cout, cin, tr, col, 8 * (g & 1) + j, 2 * m + (g >> 1));
+                val = pair && g >= 2 ? 0.f : w_elem(w, cout, cin, tr, col, 8 * (g & 1) + j, 3 * dzdy + dx);
             e[j] = f2bf(val);
         }
-        wpk[(v * 14 + m) * 64 + lane] = pack8(e);
+        wpk[(v * WV_OPS + m) * 64 + lane] = pack8(e);
     }
 }
 
 constexpr int ZPB = 4;  // z-chunks of BZ slices per workgroup (weights loaded once per workgroup)
+constexpr int SPT = (HVOX + 255) / 256;  // halo voxels staged per thread
+constexpr int SPT1 = 3;                   // of which loaded before the chunk barrier
+static_assert(SPT - SPT1 <= SPT1, "the second staging round reuses the first round's registers");
 
 // 16 -> 16 (or 16 -> 1) channel convolution of a bf16 [V][16] input.
+// A workgroup computes a 16x8x4 brick per z-chunk; wavefront w owns the
+// output rows y = 2w, 2w+1 (all four z).  The halo sits in LDS as two planes
+// of 16-byte channel halves.  One B operand holds an input row segment at x
+// shift dx (k-groups 0-1) and dx + 1 (k-groups 2-3): it is read once and
+// feeds every output row whose 3x3x3 window holds that row segment, up to six
+// MFMAs, with the weights of the tap pair from registers.  Each output row
+// accumulates its taps in tap order.
 template <int EP>
 __global__ __launch_bounds__(256) void conv16_kernel(int nx, int ny, int nz, const u32x4 *__restrict__ in,
                                                      const u32x4 *__restrict__ wpk, const float *__restrict__ bias,
@@ -164,61 +178,105 @@ __global__ __launch_bounds__(256) void conv16_kernel(int nx, int ny, int nz, con
     const int col = lane & 15, g = lane >> 4;
     const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
 
-    // A operands (weights) and B offsets of the 14 MFMAs of a row
-    u32x4 A[14];
-    int off[14];
+    u32x4 A[WV_OPS];
 #pragma unroll
-    for (int m = 0; m < 14; ++m) {
+    for (int m = 0; m < WV_OPS; ++m)
         A[m] = wpk[m * 64 + lane];
-        const int t = 2 * m + (g >> 1);
-        const int tt = t < 27 ? t : 13;  // the padding tap has zero weights
-        off[m] = (g & 1) * HVOX + (tap_dz(tt) + 1) * HYX + (tap_dy(tt) + 1) * HX + (tap_dx(tt) + 1) + col;
-    }
+    // this lane's B address for halo plane 0, halo row 2w, pair 0
+    const int bbase = (g & 1) * HVOX + (2 * wv) * HX + col + (g >> 1);
+    const int shift2 = 2 - (g >> 1);  // pair 1 reads dx = 2 in every k-group
 
     double lsum = 0.0;
     const int zc_end = min((int)(blockIdx.z + 1) * ZPB, (nz + BZ - 1) / BZ);
     for (int zc = blockIdx.z * ZPB; zc < zc_end; ++zc) {
         const int z0 = zc * BZ;
-        __syncthreads();  // the previous chunk's operands are consumed
-        // one voxel (both 16-byte channel halves) per thread: contiguous
-        // 32-byte global reads, consecutive 16-byte LDS writes per plane
-        for (int q = tid; q < HVOX; q += 256) {
+        // The halo's z-slabs as one buffer resource (at most six slabs; the
+        // launcher checks they span under 4 GiB): a voxel outside the volume
+        // reads at the resource's end and loads zeros.  Every load of the
+        // chunk is issued before the barrier, so the loads overlap the other
+        // wavefronts' work on the previous chunk.
+        const int zlo = max(z0 - 1, 0), zhi = min(z0 + BZ, nz - 1);
+        const unsigned span = (unsigned)((unsigned long long)(zhi - zlo + 1) * nx * ny * 32ull);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(in + (size_t)zlo * nx * ny * 2), (short)0, (int)span, 0x00020000);
+        auto load = [&](int k, u32x4 &a, u32x4 &b) {
+            const int q = tid + 256 * k;
             const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
             const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
-            u32x4 v0 = {0u, 0u, 0u, 0u}, v1 = {0u, 0u, 0u, 0u};
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-                const size_t vi = vidx(gx, gy, gz, nx, ny) * 2;
-                v0 = in[vi];
-                v1 = in[vi + 1];
+            const bool ok = q < HVOX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+            const unsigned off =
+                ok ? (((unsigned)(gz - zlo) * (unsigned)ny + (unsigned)gy) * (unsigned)nx + (unsigned)gx) * 32u : span;
+            a = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0));
+            b = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(off + 16u), 0, 0));
+        };
+        auto store = [&](int k, const u32x4 &a, const u32x4 &b) {
+            const int q = tid + 256 * k;
+            if (q < HVOX) {
+                lds[q] = a;
+                lds[HVOX + q] = b;
             }
-            lds[q] = v0;
-            lds[HVOX + q] = v1;
-        }
-        __syncthreads();
-        for (int r = wv; r < BY * BZ; r += 4) {
-            const int ry = r % BY, rz = r / BY;
-            const int base = rz * HYX + ry * HX;
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        };
+        // the first SPT1 voxels' loads are in flight across the barrier; the
+        // rest follow (fewer registers held)
+        u32x4 h0[SPT1], h1[SPT1];
 #pragma unroll
-            for (int m = 0; m < 14; ++m)
-                acc = mfma(A[m], lds[base + off[m]], acc);
-            const int gx = x0 + col, gy = y0 + ry, gz = z0 + rz;
-            if (gx >= nx || gy >= ny || gz >= nz)
-                continue;
-            const size_t v = vidx(gx, gy, gz, nx, ny);
-            if (EP == EP_RELU || EP == EP_MASK) {
-                store4<EP>(acc, g, v, bias, out16, mask);
-            } else if (g == 0) {
-                const float z4 = acc[0] + bias[0];
-                if (EP == EP_OUT) {
-                    outf[v] = x[v] + z4 / s;
-                } else {
-                    const float res = (s * x[v] + z4) - s * target[v];
-                    outf[v] = res * gscale;
-                    lsum += (double)res * (double)res;
+        for (int k = 0; k < SPT1; ++k)
+            load(k, h0[k], h1[k]);
+        __syncthreads();  // the previous chunk's operands are consumed
+#pragma unroll
+        for (int k = 0; k < SPT1; ++k)
+            store(k, h0[k], h1[k]);
+#pragma unroll
+        for (int k = SPT1; k < SPT; ++k)
+            load(k, h0[k - SPT1], h1[k - SPT1]);
+#pragma unroll
+        for (int k = SPT1; k < SPT; ++k)
+            store(k, h0[k - SPT1], h1[k - SPT1]);
+        __syncthreads();
+        f32x4 acc[2][BZ];
+#pragma unroll
+        for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+            for (int jz = 0; jz < BZ; ++jz)
+                acc[jy][jz] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // halo plane zp (output z = zp - 1 + dz), halo row 2w + yp
+#pragma unroll
+        for (int zp = 0; zp < BZ + 2; ++zp)
+#pragma unroll
+            for (int yp = 0; yp < 4; ++yp)
+#pragma unroll
+                for (int pair = 0; pair < 2; ++pair) {
+                    const u32x4 b = lds[bbase + zp * HYX + yp * HX + (pair ? shift2 : 0)];
+#pragma unroll
+                    for (int jz = 0; jz < BZ; ++jz)
+#pragma unroll
+                        for (int jy = 0; jy < 2; ++jy) {
+                            const int tz = zp - jz, ty = yp - jy;
+                            if (tz >= 0 && tz <= 2 && ty >= 0 && ty <= 2)
+                                acc[jy][jz] = mfma(A[2 * (3 * tz + ty) + pair], b, acc[jy][jz]);
+                        }
+                }
+#pragma unroll
+        for (int jz = 0; jz < BZ; ++jz)
+#pragma unroll
+            for (int jy = 0; jy < 2; ++jy) {
+                const int gx = x0 + col, gy = y0 + 2 * wv + jy, gz = z0 + jz;
+                if (gx >= nx || gy >= ny || gz >= nz)
+                    continue;
+                const size_t v = vidx(gx, gy, gz, nx, ny);
+                if (EP == EP_RELU || EP == EP_MASK) {
+                    store4<EP>(acc[jy][jz], g, v, bias, out16, mask);
+                } else if (g == 0) {
+                    const float z4 = acc[jy][jz][0] + bias[0];
+                    if (EP == EP_OUT) {
+                        outf[v] = x[v] + z4 / s;
+                    } else {
+                        const float res = (s * x[v] + z4) - s * target[v];
+                        outf[v] = res * gscale;
+                        lsum += (double)res * (double)res;
+                    }
                 }
             }
-        }
     }
     if (EP == EP_LOSS) {
         // fixed-order block reduction of the squared residuals
@@ -258,14 +316,30 @@ __global__ __launch_bounds__(256) void conv1_kernel(int nx, int ny, int nz, cons
     const int zc_end = min((int)(blockIdx.z + 1) * ZPB, (nz + BZ - 1) / BZ);
     for (int zc = blockIdx.z * ZPB; zc < zc_end; ++zc) {
         const int z0 = zc * BZ;
-        __syncthreads();
-        for (int q = tid; q < HVOX; q += 256) {
+        // the halo's z-slabs as one buffer resource (reads outside the
+        // volume load zeros); every load is issued before the barrier
+        const int zlo = max(z0 - 1, 0), zhi = min(z0 + BZ, nz - 1);
+        const unsigned span = (unsigned)((unsigned long long)(zhi - zlo + 1) * nx * ny * 4ull);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(in + (size_t)zlo * nx * ny), (short)0, (int)span, 0x00020000);
+        float hv[SPT];
+        bool hok[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int q = tid + 256 * k;
             const int lx = q % HX, ly = (q / HX) % HY, lz = q / HYX;
             const int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z0 + lz - 1;
-            unsigned int v = 0u;
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-                v = f2bf(sc * in[vidx(gx, gy, gz, nx, ny)]);
-            lds[q] = (unsigned short)v;
+            hok[k] = q < HVOX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+            const unsigned off =
+                hok[k] ? (((unsigned)(gz - zlo) * (unsigned)ny + (unsigned)gy) * (unsigned)nx + (unsigned)gx) * 4u : span;
+            hv[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int q = tid + 256 * k;
+            if (q < HVOX)
+                lds[q] = (unsigned short)(hok[k] ? f2bf(sc * hv[k]) : 0u);
         }
         __syncthreads();
         for (int r = wv; r < BY * BZ; r += 4) {
@@ -520,7 +594,7 @@ Layout layout(int nx, int ny, int nz, bool training)
         o += align256(bytes);
         return r;
     };
-    L.wpk = take((size_t)WV_COUNT * 14 * 64 * 16);
+    L.wpk = take((size_t)WV_COUNT * WV_OPS * 64 * 16);
     L.a1 = take(V * 32);
     L.a2 = take(V * 32);
     if (training) {
@@ -544,6 +618,12 @@ kv_status check_dims(int nx, int ny, int nz)
     if (nx < 1 || ny < 1 || nz < 1 || nx > (1 << 24) || ny > (1 << 24) || nz > (1 << 24) ||
         (long long)nx * ny * nz > (1LL << 30)) {
         kv_set_error("bad CNN volume size %d x %d x %d", nx, ny, nz);
+        return KV_ERR_INVALID;
+    }
+    // conv16_kernel addresses the BZ + 2 slices of a chunk's halo (32 bytes
+    // per voxel) through one buffer resource with 32-bit offsets
+    if ((unsigned long long)nx * ny * 32ull * (BZ + 2) + 32ull > 0xFFFFFFFFull) {
+        kv_set_error("CNN slices of %d x %d voxels exceed the %d-voxel limit", nx, ny, (int)(0xFFFFFFFFull / 192));
         return KV_ERR_INVALID;
     }
     return KV_OK;
@@ -638,7 +718,7 @@ kv_status kv_cnn_forward(kv_context *ctx, const float *d_params, float scale, co
     hipStream_t s = ctx->stream;
     auto *a1 = at<unsigned short>(d_work, L.a1), *a2 = at<unsigned short>(d_work, L.a2);
     u32x4 *wpk = at<u32x4>(d_work, L.wpk);
-    auto W = [&](int v) { return wpk + v * 14 * 64; };
+    auto W = [&](int v) { return wpk + v * WV_OPS * 64; };
     auto c16 = [](const unsigned short *q) { return reinterpret_cast<const u32x4 *>(q); };
     const float *p = d_params;
     pack_weights_kernel<<<WV_COUNT, 64, 0, s>>>(p, wpk, P_W1, P_W2, P_W3, P_W4);
@@ -677,7 +757,7 @@ kv_status kv_cnn_gradient(kv_context *ctx, const float *d_params, float scale, c
     const float gscale = (float)(2.0 / nvox);
     auto c16 = [](const unsigned short *q) { return reinterpret_cast<const u32x4 *>(q); };
     u32x4 *wpk = at<u32x4>(d_work, L.wpk);
-    auto W = [&](int v) { return wpk + v * 14 * 64; };
+    auto W = [&](int v) { return wpk + v * WV_OPS * 64; };
 
     pack_weights_kernel<<<WV_COUNT, 64, 0, s>>>(p, wpk, P_W1, P_W2, P_W3, P_W4);
     conv1_kernel<EP_RELU><<<grid, 256, 0, s>>>(nx, ny, nz, d_x, scale, W(WV_L1), p + P_B1, a1, nullptr);
