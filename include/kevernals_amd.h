@@ -147,6 +147,13 @@ kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *sr
  * Ax the mode-A forward projection of d_vol; reduced deterministically in
  * FP64.  Synchronous (writes *out on return). */
 kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d_vol, double *out);
+/* kv_log_likelihood of n <= KV_LL_MAX volumes (out[i] for d_vols[i], each
+ * bit-identical to a kv_log_likelihood call) with one ray walk for all of
+ * them: the likelihood gates compare up to three candidates.  Uses two more
+ * projection-sized buffers of the context (allocated on first use).
+ * Synchronous. */
+#define KV_LL_MAX 3
+kv_status kv_log_likelihoods(kv_context *ctx, const float *d_data, const float *const *d_vols, int n, double *out);
 
 /* Mode-A sensitivity s = A^T 1 into d_vol. */
 kv_status kv_sensitivity(kv_context *ctx, float *d_vol);

@@ -77,6 +77,7 @@ _SIGS = {
     "kv_sart_iteration": (_I, [_P, _P, _P, _F]),
     "kv_shift_and_add": (_I, [_P, _P, _P, _I, _P, _F, _F, _F, _P]),
     "kv_log_likelihood": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+    "kv_log_likelihoods": (_I, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "kv_sensitivity": (_I, [_P, _P]),
     "kv_trace_rays": (_I, [_P, _I, _I, _P, _I, _P, _P, _P]),
     "kv_forward_project_host": (_I, [_P, _P, _P, _I]),
@@ -151,6 +152,7 @@ def _nbytes(x) -> int:
 
 
 CNN_NPARAMS = 14737
+LL_MAX = 3  # volumes per log_likelihoods call
 TRAJ_MAX = 32
 
 
@@ -312,6 +314,15 @@ class Context:
         out = ctypes.c_double()
         self._dev(self.lib.kv_log_likelihood, _ptr(data), _ptr(vol), ctypes.byref(out))
         return out.value
+
+    def log_likelihoods(self, data, vols) -> list:
+        """log_likelihood of up to LL_MAX volumes with one ray walk (each
+        value bit-identical to log_likelihood of that volume)."""
+        n = len(vols)
+        ptrs = (ctypes.c_void_p * n)(*[_ptr(v) for v in vols])
+        out = (ctypes.c_double * n)()
+        self._dev(self.lib.kv_log_likelihoods, _ptr(data), ptrs, n, out)
+        return list(out)
 
     def trace_rays(self, rays, cap: int = 1024, mode: int = MODE_R):
         rays = np.ascontiguousarray(rays, dtype=np.int32)

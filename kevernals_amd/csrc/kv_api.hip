@@ -155,6 +155,7 @@ void release_geometry(kv_context *ctx)
     free_tables(ctx->tab_a);
     free_dev(ctx->d_ratio);
     free_dev(ctx->d_sens);
+    free_dev(ctx->d_ll);
     ctx->tab_r_ok = ctx->tab_a_ok = ctx->sens_ok = false;
     ctx->has_geom = false;
 }
@@ -649,6 +650,48 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
     for (int i = 0; i < nb; ++i)
         s += part[i];
     *out = s;
+    return KV_OK;
+}
+
+kv_status kv_log_likelihoods(kv_context *ctx, const float *d_data, const float *const *d_vols, int n, double *out)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    if (!d_data || !d_vols || !out || n < 1 || n > KV_LL_MAX) {
+        kv_set_error("log-likelihoods need 1..%d volumes", KV_LL_MAX);
+        return KV_ERR_INVALID;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!d_vols[i]) {
+            kv_set_error("null volume");
+            return KV_ERR_INVALID;
+        }
+    }
+    const int nb = 1024;
+    if (ctx->red_cap < nb) {
+        free_dev(ctx->d_red);
+        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
+            return st;
+        ctx->red_cap = nb;
+    }
+    const size_t nray = (size_t)kv_nray(ctx->g);
+    if (n > 1 && !ctx->d_ll) {
+        if (kv_status st = dev_alloc((void **)&ctx->d_ll, (KV_LL_MAX - 1) * nray * sizeof(float)))
+            return st;
+    }
+    float *outs[KV_LL_MAX] = {ctx->d_ratio, ctx->d_ll, ctx->d_ll ? ctx->d_ll + nray : nullptr};
+    KV_TRY_HIP(kv_launch_fp_a_multi(ctx->g, n, d_vols, outs, ctx->stream));
+    std::vector<double> part(nb);
+    for (int i = 0; i < n; ++i) {
+        // the same reduction as kv_log_likelihood
+        KV_TRY_HIP(kv_launch_loglik(ctx->g, d_data, outs[i], ctx->d_red, nb, ctx->stream));
+        KV_TRY_HIP(hipMemcpyAsync(part.data(), ctx->d_red, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KV_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (int b = 0; b < nb; ++b)
+            sum += part[b];
+        out[i] = sum;
+    }
     return KV_OK;
 }
 

@@ -532,6 +532,44 @@ __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, KvTables t, const f
     fp_a_ray<EP>(g, t, vol, out, data, p, u, v);
 }
 
+// Mode-A forward projections of NV volumes in one walk per ray: each
+// volume's sum takes the same products and additions in the same order as
+// fp_a_ray, so every projection is bit-identical to a single-volume pass.
+struct KvMulti {
+    const float *vol[KV_LL_MAX];
+    float *out[KV_LL_MAX];
+};
+
+template <int NV>
+__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m)
+{
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (u >= g.nu || v >= g.nv)
+        return;
+    RayA r;
+    ray_setup_a(g, p, u, v, r);
+    __amdgpu_buffer_rsrc_t vrs[NV];
+    float acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        vrs[i] = volume_rsrc(g, m.vol[i]);
+        acc[i] = 0.f;
+    }
+    walk_a(r, [&](bool ok, unsigned k, float len) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs[i], (int)(k << 2), 0, 0));
+            acc[i] = ok ? acc[i] + len * val : acc[i];
+        }
+    });
+    const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        m.out[i][o] = r.L * acc[i];
+}
+
 // Persistent mode-A forward pass (MLEM ratio) over a batch, as
 // fp_r_batch_kernel.
 __global__ __launch_bounds__(256) void fp_a_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
@@ -607,6 +645,20 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
     case KV_EP_MLEM: fp_r_kernel<KV_EP_MLEM><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
     case KV_EP_COUNT: fp_r_kernel<KV_EP_COUNT><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
     default: fp_r_kernel<KV_EP_SART><<<grid, block, 0, s>>>(g, vol, out, data, relax); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *const *outs, hipStream_t s)
+{
+    KvMulti m{};
+    for (int i = 0; i < n; ++i)
+        m.vol[i] = vols[i], m.out[i] = outs[i];
+    const dim3 grid((g.nu + 63) / 64, (g.nv + 3) / 4, g.np), block(256);
+    switch (n) {
+    case 1: fp_a_multi_kernel<1><<<grid, block, 0, s>>>(g, m); break;
+    case 2: fp_a_multi_kernel<2><<<grid, block, 0, s>>>(g, m); break;
+    default: fp_a_multi_kernel<3><<<grid, block, 0, s>>>(g, m); break;
     }
     return hipGetLastError();
 }

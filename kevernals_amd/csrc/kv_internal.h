@@ -86,6 +86,7 @@ struct kv_context {
     bool sens_ok = false;
     double *d_red = nullptr;   // partial sums of deterministic reductions
     int red_cap = 0;
+    float *d_ll = nullptr;     // projections of the further volumes of kv_log_likelihoods
 };
 
 // Thread-local error message.
@@ -125,6 +126,8 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
 // Mode-A forward pass.  KV_EP_MLEM writes the MLEM ratio data/Ax
 // premultiplied by the ray's length factor from the mode-A tables t (the
 // input the mode-A backprojector expects); the other epilogues ignore t.
+// Mode-A forward projections of n <= KV_LL_MAX volumes in one walk per ray.
+hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *const *outs, hipStream_t s);
 hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, float *out, int epilogue,
                           const float *data, hipStream_t s);
 hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx,

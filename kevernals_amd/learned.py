@@ -170,8 +170,7 @@ def gated_mlem(ctx: Context, cnn: Optional[Cnn], data, iters: int, every: int = 
         active = cnn is not None and (patience is None or misses < patience)
         if active and k >= first and (k - first) % every == 0:
             cnn.forward(x, cand)
-            ll_x = ctx.log_likelihood(data, x)
-            ll_c = ctx.log_likelihood(data, cand)
+            ll_x, ll_c = ctx.log_likelihoods(data, [x, cand])
             if ll_c >= ll_x:
                 x.copy_(cand)
                 run.accepted += 1
@@ -261,8 +260,7 @@ def batched_gated_mlem(batch, iters: int, params: Optional[np.ndarray] = None, s
             fetch(i)
             cnn.forward(views[i], cand[i])
             data_addr = bufs[i][1]
-            ll_x = ctxs[i].log_likelihood(data_addr, views[i])
-            ll_c = ctxs[i].log_likelihood(data_addr, cand[i])
+            ll_x, ll_c = ctxs[i].log_likelihoods(data_addr, [views[i], cand[i]])
             if ll_c >= ll_x:
                 ctxs[i].copy(bufs[i][0], cand[i], cand[i].numel() * 4)
                 run.accepted[i] += 1
@@ -280,13 +278,14 @@ def _chain_gate(ctx, data, x, y, y2=None):
     """The likelihood gate of the final steps: the jump x -> y is kept if
     log L(y) >= log L(x); then the CNN step y -> y2 if log L(y2) >= log L(y).
     Returns (steps kept: 0, 1 or 2, log-likelihood of the kept volume)."""
-    ll_x = ctx.log_likelihood(data, x)
-    ll_y = ctx.log_likelihood(data, y)
+    # one ray walk for every candidate (values as separate evaluations)
+    lls = ctx.log_likelihoods(data, [x, y] if y2 is None else [x, y, y2])
+    ll_x, ll_y = lls[0], lls[1]
     if ll_y < ll_x:
         return 0, ll_x
     if y2 is None:
         return 1, ll_y
-    ll_y2 = ctx.log_likelihood(data, y2)
+    ll_y2 = lls[2]
     return (2, ll_y2) if ll_y2 >= ll_y else (1, ll_y)
 
 

@@ -280,6 +280,29 @@ def test_mlem_a_against_fp64_oracle_and_loglik(K, O):
     assert all(b >= a - 1e-7 * abs(a) for a, b in zip(lls, lls[1:]))
 
 
+def test_log_likelihoods_share_one_walk_bit_for_bit(K, O):
+    """log_likelihoods of 1..3 volumes (one ray walk) equals log_likelihood
+    of each volume bit for bit, and each matches the FP64 oracle."""
+    g = half_small()
+    o = O.Oracle(g)
+    from kevernals_amd.phantom import poisson_line_integrals
+
+    data = poisson_line_integrals(o.fp_a(phantom_of(g, 4)), 1e4, 5)
+    ctx = K.Context(0, g)
+    d = dev(data)
+    vols = [dev(phantom_of(g, s) + np.float32(0.001)) for s in (5, 6, 7)]
+    single = [ctx.log_likelihood(d, v) for v in vols]
+    for n in (1, 2, 3):
+        assert ctx.log_likelihoods(d, vols[:n]) == single[:n]
+    assert ctx.log_likelihoods(d, [vols[2], vols[0]]) == [single[2], single[0]]
+    for v, ll in zip(vols, single):
+        ref = o.loglik_a64(data.astype(np.float64), host(v).astype(np.float64))
+        assert abs(ll - ref) <= 1e-5 * abs(ref)
+    for bad in ([], vols + vols[:1]):
+        with pytest.raises(RuntimeError):
+            ctx.log_likelihoods(d, bad)
+
+
 # ---------------------------------------------------------- shift-and-add --
 def test_shift_and_add_bit_exact(K, O):
     g = small()
