@@ -369,7 +369,7 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
     if (stored < 1 || stored > alloc || high >= alloc || high + 1 < stored)
         return fail("inconsistent BitsStored / HighBit");
     const size_t bpp = (size_t)alloc / 8, count = (size_t)rows * cols * frames;
-    if ((size_t)pixels.length < count * bpp)
+    if ((size_t)pixels.length / bpp < count)  // (no overflow for any header values)
         return fail("pixel data shorter than Rows x Columns x NumberOfFrames");
 
     f.rows = rows;
