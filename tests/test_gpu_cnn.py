@@ -44,10 +44,10 @@ def ulp_diff(a, b):
     return np.abs(ia - ib)
 
 
-def inputs(seed=0):
+def inputs(seed=0, shape=SHAPE):
     r = rng(100 + seed)
-    x = (0.02 + 0.06 * r.random(SHAPE)).astype(np.float32)
-    t = (x + 0.01 * r.standard_normal(SHAPE)).astype(np.float32)
+    x = (0.02 + 0.06 * r.random(shape)).astype(np.float32)
+    t = (x + 0.01 * r.standard_normal(shape)).astype(np.float32)
     from oracle import cnn
 
     p = cnn.init_params(seed).astype(np.float64)
@@ -220,3 +220,24 @@ def test_short_training_lowers_loss(K, oracle_mod):
         ctx.cnn_adam(params, grad, m, v, step, 1e-3)
     print(f"[cnn] training loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
     assert losses[-1] < 0.5 * losses[0]  # the FP64 oracle run of the same loop reaches 0.18x
+
+
+@pytest.mark.parametrize("shape", [(1, 5, 7), (4, 8, 16), (5, 9, 70), (9, 1, 3), (2, 17, 1)])
+def test_forward_at_volume_edges(K, oracle_mod, shape):
+    """Volumes thinner than a brick in every direction, one slice, one row,
+    one column: halo voxels outside the volume read as zero (the staging's
+    out-of-range buffer loads) and partial bricks are stored exactly."""
+    from oracle import cnn
+
+    x, _, params = inputs(3, shape)
+    ctx = K.Context(0)
+    work = torch.empty(K.cnn_workspace_size(shape, False), dtype=torch.uint8, device="cuda")
+    y = torch.full(shape, float("nan"), device="cuda")
+    ctx.cnn_forward(torch.from_numpy(params).cuda(), SCALE, torch.from_numpy(x).cuda(), y, work)
+    got = y.cpu().numpy().astype(np.float64)
+    assert np.isfinite(got).all()
+    ref_bf, _ = cnn.forward(params, x, SCALE, bf16_round=True)
+    corr = np.abs(ref_bf - x)
+    err = np.abs(got - ref_bf)
+    print(f"[cnn] {shape}: max err {err.max():.3e} of max correction {corr.max():.3e}", flush=True)
+    assert err.max() <= 5e-3 * corr.max()
