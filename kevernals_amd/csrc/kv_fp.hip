@@ -487,6 +487,33 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
     }
 }
 
+// Sum over a mode-A walk of len * vol[voxel] for the accepted segments, the
+// value of acc = ok ? acc + len * v : acc, with no branch around the load:
+// a rejected segment loads at the volume's end (zero) with weight zero, and
+// each product is added one step later, so that the load is in flight
+// while the next step is walked.  acc starts at +0 and never becomes -0,
+// so adding a rejected segment's +0 leaves it unchanged: the sum is the
+// select form's bit for bit (for finite volumes; a volume of exactly 2^30
+// voxels reads voxel 0 for rejected segments, again times zero).
+struct AccA {
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned oob;  // first index past the volume
+    float acc = 0.f, pw = 0.f, pv = 0.f;
+    __device__ __forceinline__ void init(const KvGeom &g, const float *vol)
+    {
+        rs = volume_rsrc(g, vol);
+        oob = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
+    }
+    __device__ __forceinline__ void visit(bool ok, unsigned k, float len)
+    {
+        const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((ok ? k : oob) << 2), 0, 0));
+        acc = acc + pw * pv;
+        pw = ok ? len : 0.f;
+        pv = v;
+    }
+    __device__ __forceinline__ float sum() const { return acc + pw * pv; }
+};
+
 // One ray of the mode-A forward pass with its epilogue.
 template <int EP>
 __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const KvTables &t, const float *__restrict__ vol,
@@ -501,13 +528,10 @@ __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const KvTables &t, con
         out[o] = (float)c;
         return;
     }
-    float acc = 0.f;
-    const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
-    walk_a(r, [&](bool ok, unsigned k, float len) {
-        const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
-        acc = ok ? acc + len * val : acc;
-    });
-    const float ax = r.L * acc;
+    AccA a;
+    a.init(g, vol);
+    walk_a(r, [&](bool ok, unsigned k, float len) { a.visit(ok, k, len); });
+    const float ax = r.L * a.sum();
     if (EP == KV_EP_PLAIN) {
         out[o] = ax;
     } else {
@@ -550,24 +574,19 @@ __global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m)
         return;
     RayA r;
     ray_setup_a(g, p, u, v, r);
-    __amdgpu_buffer_rsrc_t vrs[NV];
-    float acc[NV];
+    AccA a[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        vrs[i] = volume_rsrc(g, m.vol[i]);
-        acc[i] = 0.f;
-    }
+    for (int i = 0; i < NV; ++i)
+        a[i].init(g, m.vol[i]);
     walk_a(r, [&](bool ok, unsigned k, float len) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs[i], (int)(k << 2), 0, 0));
-            acc[i] = ok ? acc[i] + len * val : acc[i];
-        }
+        for (int i = 0; i < NV; ++i)
+            a[i].visit(ok, k, len);
     });
     const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-        m.out[i][o] = r.L * acc[i];
+        m.out[i][o] = r.L * a[i].sum();
 }
 
 // Persistent mode-A forward pass (MLEM ratio) over a batch, as
