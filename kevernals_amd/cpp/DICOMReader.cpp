@@ -2,7 +2,10 @@
 // a DICOM parser of its own: PS3.5 data encoding, PS3.10 file format).
 #include "DICOMReader.h"
 
-#include <omp.h>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -185,20 +188,6 @@ std::string text(const Parser &ps, const Element &e)
     return s;
 }
 
-bool read_all(const std::string &path, std::vector<uint8_t> *out)
-{
-    std::ifstream f(path, std::ios::binary);
-    if (!f)
-        return false;
-    f.seekg(0, std::ios::end);
-    const std::streamoff n = f.tellg();
-    if (n < 0)
-        return false;
-    f.seekg(0);
-    out->resize((size_t)n);
-    return n == 0 || (bool)f.read(reinterpret_cast<char *>(out->data()), n);
-}
-
 bool inflate_raw(const uint8_t *p, size_t n, std::vector<uint8_t> *out)
 {
     z_stream z{};
@@ -224,13 +213,13 @@ bool inflate_raw(const uint8_t *p, size_t n, std::vector<uint8_t> *out)
 }
 
 // Start of the dataset: after the Part 10 preamble, or 0 for a bare dataset.
-bool locate(const std::vector<uint8_t> &b, size_t *start)
+bool locate(const uint8_t *b, size_t n, size_t *start)
 {
-    if (b.size() >= 132 && std::memcmp(b.data() + 128, "DICM", 4) == 0) {
+    if (n >= 132 && std::memcmp(b + 128, "DICM", 4) == 0) {
         *start = 132;
         return true;
     }
-    if (b.size() >= 8) {
+    if (n >= 8) {
         const uint16_t g = (uint16_t)(b[0] | b[1] << 8);
         if (g == 0x0002 || g == 0x0008) {
             *start = 0;
@@ -251,28 +240,77 @@ bool CanReadFile(const std::string &path)
     f.read(reinterpret_cast<char *>(head.data()), 132);
     head.resize((size_t)f.gcount());
     size_t start;
-    return locate(head, &start);
+    return locate(head.data(), head.size(), &start);
 }
 
-bool ReadFile(const std::string &path, Frames *out, std::string *err)
+namespace {
+
+// A file's bytes, memory-mapped read-only (no copy, no fresh memory).
+struct Mapped {
+    const uint8_t *p = nullptr;
+    size_t n = 0;
+    void *map = nullptr;
+    Mapped() = default;
+    Mapped(const Mapped &) = delete;
+    Mapped &operator=(const Mapped &) = delete;
+    ~Mapped()
+    {
+        if (map)
+            munmap(map, n);
+    }
+    bool open(const std::string &path)
+    {
+        const int fd = ::open(path.c_str(), O_RDONLY);
+        if (fd < 0)
+            return false;
+        struct stat st;
+        const bool ok = fstat(fd, &st) == 0 && st.st_size >= 0;
+        if (ok && st.st_size > 0) {
+            map = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (map == MAP_FAILED) {
+                map = nullptr;
+            } else {
+                p = static_cast<const uint8_t *>(map);
+                n = (size_t)st.st_size;
+            }
+        }
+        ::close(fd);
+        return ok && (st.st_size == 0 || map);
+    }
+};
+
+// One file parsed up to its pixel cells (not decoded).
+struct Parsed {
+    Mapped file;
+    std::vector<uint8_t> inflated;  // the dataset of a deflated file
+    const uint8_t *cells = nullptr;
+    bool big = false;
+    int rows = 0, cols = 0, frames = 0, bpp = 0, stored = 0, shift = 0, rep = 0;
+    double spacing[2] = {1.0, 1.0};
+    bool hasPatientID = false;
+    std::string acquisitionTime;
+};
+
+bool Parse(const std::string &path, Parsed *f, std::string *err)
 {
     auto fail = [&](const std::string &m) {
         if (err)
             *err = path + ": " + m;
         return false;
     };
-    std::vector<uint8_t> bytes;
-    if (!read_all(path, &bytes))
+    if (!f->file.open(path))
         return fail("cannot read the file");
+    const uint8_t *bytes = f->file.p;
+    const size_t nbytes = f->file.n;
     size_t start;
-    if (!locate(bytes, &start))
+    if (!locate(bytes, nbytes, &start))
         return fail("not a DICOM file");
 
     // File meta information (group 0002): explicit VR little endian.
     Syntax syntax;
     syntax.explicitVR = true;
     {
-        Parser meta(bytes.data(), bytes.size(), Syntax{});
+        Parser meta(bytes, nbytes, Syntax{});
         meta.seek(start);
         std::string ts;
         while (meta.pos() + 2 <= meta.size() && meta.u16(meta.pos()) == 0x0002) {
@@ -287,7 +325,7 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
         start = meta.pos();
         if (ts.empty()) {
             // bare dataset: implicit VR little endian unless a VR is present
-            const bool vr = bytes.size() >= start + 6 && bytes[start + 4] >= 'A' && bytes[start + 4] <= 'Z' &&
+            const bool vr = nbytes >= start + 6 && bytes[start + 4] >= 'A' && bytes[start + 4] <= 'Z' &&
                             bytes[start + 5] >= 'A' && bytes[start + 5] <= 'Z';
             syntax.explicitVR = vr;
         } else if (ts == "1.2.840.10008.1.2") {
@@ -301,14 +339,13 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
             return fail("transfer syntax " + ts + " (encapsulated pixel data) is not supported");
         }
     }
-    std::vector<uint8_t> inflated;
-    const uint8_t *ds = bytes.data() + start;
-    size_t dsn = bytes.size() - start;
+    const uint8_t *ds = bytes + start;
+    size_t dsn = nbytes - start;
     if (syntax.deflated) {
-        if (!inflate_raw(ds, dsn, &inflated))
+        if (!inflate_raw(ds, dsn, &f->inflated))
             return fail("deflated dataset does not inflate");
-        ds = inflated.data();
-        dsn = inflated.size();
+        ds = f->inflated.data();
+        dsn = f->inflated.size();
     }
 
     Parser p(ds, dsn, syntax);
@@ -317,7 +354,6 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
     bool hasPixelSpacing = false, hasImagerSpacing = false;
     Element pixels;
     bool havePixels = false;
-    Frames f;
     auto us = [&](const Element &e) { return e.length >= 2 ? (int)p.u16(e.value) : -1; };
     auto ds2 = [&](const Element &e, double v[2]) {
         const std::string s(reinterpret_cast<const char *>(ds + e.value), e.length);
@@ -329,8 +365,8 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
             return fail(p.error());
         const uint32_t tag = (uint32_t)e.group << 16 | e.element;
         switch (tag) {
-        case 0x00100020: f.hasPatientID = true; break;
-        case 0x00080032: if (e.length != kUndefined) f.acquisitionTime = text(p, e); break;
+        case 0x00100020: f->hasPatientID = true; break;
+        case 0x00080032: if (e.length != kUndefined) f->acquisitionTime = text(p, e); break;
         case 0x00280002: spp = us(e); break;
         case 0x00280008: if (e.length != kUndefined) frames = std::atoi(text(p, e).c_str()); break;
         case 0x00280010: rows = us(e); break;
@@ -371,38 +407,94 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err)
     const size_t bpp = (size_t)alloc / 8, count = (size_t)rows * cols * frames;
     if ((size_t)pixels.length / bpp < count)  // (no overflow for any header values)
         return fail("pixel data shorter than Rows x Columns x NumberOfFrames");
-
-    f.rows = rows;
-    f.columns = cols;
-    f.frames = frames;
+    f->cells = ds + pixels.value;
+    f->big = syntax.bigEndian;
+    f->rows = rows;
+    f->cols = cols;
+    f->frames = frames;
+    f->bpp = (int)bpp;
+    f->stored = stored;
+    f->shift = high + 1 - stored;
+    f->rep = rep;
     const double *sp = hasPixelSpacing ? pixelSpacing : hasImagerSpacing ? imagerSpacing : nullptr;
     if (sp) {
-        f.spacing[0] = sp[1];  // column spacing: x
-        f.spacing[1] = sp[0];  // row spacing: y
+        f->spacing[0] = sp[1];  // column spacing: x
+        f->spacing[1] = sp[0];  // row spacing: y
     }
-    f.values.resize(count);
-    const uint8_t *px = ds + pixels.value;
-    const int shift = high + 1 - stored;
-    const uint64_t mask = stored == 32 ? 0xFFFFFFFFull : ((1ull << stored) - 1);
-    const bool big = syntax.bigEndian;
-#pragma omp parallel for if (count >= (1u << 20) && !omp_in_parallel()) schedule(static)
-    for (size_t k = 0; k < count; ++k) {
-        uint32_t raw;
-        const uint8_t *q = px + k * bpp;
-        if (bpp == 1)
-            raw = q[0];
-        else if (bpp == 2)
-            raw = big ? (uint32_t)(q[0] << 8 | q[1]) : (uint32_t)(q[0] | q[1] << 8);
-        else
-            raw = big ? (uint32_t)q[0] << 24 | (uint32_t)q[1] << 16 | (uint32_t)q[2] << 8 | q[3]
-                      : (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24;
-        const uint64_t v = ((uint64_t)raw >> shift) & mask;
-        if (rep == 1 && (v >> (stored - 1)) & 1)
-            f.values[k] = (float)((int64_t)v - ((int64_t)1 << stored));
-        else
-            f.values[k] = (float)v;
+    return true;
+}
+
+// Stored values of nx cells of one file row (first file row = 0) from
+// column x0: the BitsStored bits ending at HighBit, sign-extended when
+// PixelRepresentation is 1.
+void decode_row(const Parsed &f, int frame, int file_row, int x0, int nx, float *dst)
+{
+    const uint8_t *q = f.cells + (((size_t)frame * f.rows + file_row) * f.cols + x0) * f.bpp;
+    const uint64_t mask = f.stored == 32 ? 0xFFFFFFFFull : ((1ull << f.stored) - 1);
+    const int64_t sign = f.rep == 1 ? (int64_t)1 << (f.stored - 1) : 0;
+    auto put = [&](int i, uint32_t raw) {
+        const uint64_t v = ((uint64_t)raw >> f.shift) & mask;
+        dst[i] = sign && (v & (uint64_t)sign) ? (float)((int64_t)v - 2 * sign) : (float)v;
+    };
+    if (f.bpp == 1) {
+        for (int i = 0; i < nx; ++i)
+            put(i, q[i]);
+    } else if (f.bpp == 2) {
+        for (int i = 0; i < nx; ++i, q += 2)
+            put(i, f.big ? (uint32_t)(q[0] << 8 | q[1]) : (uint32_t)(q[0] | q[1] << 8));
+    } else {
+        for (int i = 0; i < nx; ++i, q += 4)
+            put(i, f.big ? (uint32_t)q[0] << 24 | (uint32_t)q[1] << 16 | (uint32_t)q[2] << 8 | q[3]
+                         : (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24);
     }
-    *out = std::move(f);
+}
+
+// vtkExtractVOI's extent: the requested region clipped to the image.
+bool crop_extent(const Parsed &f, int x0, int y0, int w, int h, int e[4])
+{
+    e[0] = std::max(x0, 0), e[1] = std::min(x0 + w - 1, f.cols - 1);
+    e[2] = std::max(y0, 0), e[3] = std::min(y0 + h - 1, f.rows - 1);
+    return w > 0 && h > 0 && e[0] <= e[1] && e[2] <= e[3];
+}
+
+// Decodes each file's frames, cropped and bottom-up, into z-slabs of out:
+// file i fills slices zoff[i] .. zoff[i+1]-1 of an [z][ny][nx] image.
+void decode_into(const std::vector<const Parsed *> &files, const std::vector<int> &zoff, const int e[4], float *out)
+{
+    const int nx = e[1] - e[0] + 1, ny = e[3] - e[2] + 1;
+    const long long rows = (long long)zoff.back() * ny;
+    std::vector<int> file_of(zoff.back());
+    for (size_t i = 0; i < files.size(); ++i)
+        for (int z = zoff[i]; z < zoff[i + 1]; ++z)
+            file_of[z] = (int)i;
+#pragma omp parallel for schedule(static)
+    for (long long r = 0; r < rows; ++r) {
+        const int z = (int)(r / ny), y = e[2] + (int)(r % ny);
+        const Parsed &f = *files[file_of[z]];
+        decode_row(f, z - zoff[file_of[z]], f.rows - 1 - y, e[0], nx, out + (size_t)r * nx);
+    }
+}
+
+}  // namespace
+
+bool ReadFile(const std::string &path, Frames *out, std::string *err)
+{
+    Parsed f;
+    if (!Parse(path, &f, err))
+        return false;
+    Frames r;
+    r.rows = f.rows;
+    r.columns = f.cols;
+    r.frames = f.frames;
+    r.spacing[0] = f.spacing[0];
+    r.spacing[1] = f.spacing[1];
+    r.hasPatientID = f.hasPatientID;
+    r.acquisitionTime = f.acquisitionTime;
+    r.values.resize((size_t)f.rows * f.cols * f.frames);
+    for (int z = 0; z < f.frames; ++z)
+        for (int y = 0; y < f.rows; ++y)
+            decode_row(f, z, y, 0, f.cols, r.values.data() + ((size_t)z * f.rows + y) * f.cols);
+    *out = std::move(r);
     return true;
 }
 
@@ -431,38 +523,31 @@ ImageDataPtr CropToImage(const Frames &f, int x0, int y0, int w, int h, std::str
     return img;
 }
 
-}  // namespace kvdicom
-
-namespace {
-Result<ImageDataPtr> reading_problem(const std::string &what)
+bool ReadImage(const std::string &path, int x0, int y0, int w, int h, ImageDataPtr *out, std::string *err)
 {
-    std::cout << what << std::endl;
-    std::cout << "DataFileReadingProblem" << std::endl;
-    return DICOMReaderErrorCode::DataFileReadingProblem;
-}
-}  // namespace
-
-Result<ImageDataPtr> DICOMReader::Read(std::string const &p_dicomFilePath) const
-{
-    if (!std::filesystem::exists(p_dicomFilePath))
-        return reading_problem("Raw projections dicom file does not exist: " + p_dicomFilePath);
-    if (!kvdicom::CanReadFile(p_dicomFilePath))
-        return reading_problem("Cannot read file: " + p_dicomFilePath);
-    if (!m_tomoGeometry)
-        return reading_problem("DICOMReader: no geometry");
-    kvdicom::Frames f;
-    std::string err;
-    if (!kvdicom::ReadFile(p_dicomFilePath, &f, &err))
-        return reading_problem("Error reading file " + err);
-    const Pixel bl = m_tomoGeometry->GetProjectionROIsBLPixelPositionOnDetector();
-    const Size2D size = m_tomoGeometry->projectionsRoisSize();
-    auto img = kvdicom::CropToImage(f, bl.x, bl.y, size.x, size.y, &err);
-    if (img == nullptr)
-        return reading_problem("Empty data for raw projections " + p_dicomFilePath + ": " + err);
-    return img;
+    Parsed f;
+    if (!CanReadFile(path)) {
+        if (err)
+            *err = "Cannot read file: " + path;
+        return false;
+    }
+    if (!Parse(path, &f, err))
+        return false;
+    int e[4];
+    if (!crop_extent(f, x0, y0, w, h, e)) {
+        if (err)
+            *err = "Empty data for raw projections " + path + ": the detector region lies outside the image";
+        return false;
+    }
+    auto img = ImageDataPtr::New();
+    img->SetExtent(e[0], e[1], e[2], e[3], 0, f.frames - 1);
+    img->SetSpacing(f.spacing[0], f.spacing[1], 1.0);
+    img->AllocateScalars(VTK_FLOAT, 1);
+    decode_into({&f}, {0, f.frames}, e, static_cast<float *>(img->GetScalarPointer()));
+    *out = img;
+    return true;
 }
 
-namespace kvdicom {
 bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indicesToRemove, int x0, int y0, int w,
                          int h, bool verbose, ImageDataPtr *out, std::string *err)
 {
@@ -482,28 +567,27 @@ bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indices
     if (ec)
         return fail("cannot list " + dir + ": " + ec.message());
     std::sort(paths.begin(), paths.end());
-    struct Entry {
-        Frames frames;
-        std::string time, path, error;
-        bool ok = false;
-    };
-    // files decode independently
-    std::vector<Entry> images(paths.size());
+    // headers parse independently (the pixel cells stay mapped, undecoded)
+    std::vector<Parsed> parsed(paths.size());
+    std::vector<std::string> errors(paths.size());
+    std::vector<char> ok(paths.size(), 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < (int)paths.size(); ++i) {
-        Entry &e = images[i];
-        e.path = paths[i];
-        if (!CanReadFile(e.path)) {
-            e.error = "Cannot read file: " + e.path;
-            continue;
-        }
-        e.ok = ReadFile(e.path, &e.frames, &e.error);
-        if (e.ok && e.frames.hasPatientID)  // DICOMReader.cpp:148-152
-            e.time = e.frames.acquisitionTime;
+        if (!CanReadFile(paths[i]))
+            errors[i] = "Cannot read file: " + paths[i];
+        else
+            ok[i] = Parse(paths[i], &parsed[i], &errors[i]);
     }
-    for (const Entry &e : images)
-        if (!e.ok)
-            return fail(e.error);
+    for (size_t i = 0; i < paths.size(); ++i)
+        if (!ok[i])
+            return fail(errors[i]);
+    struct Entry {
+        const Parsed *f;
+        std::string time, path;
+    };
+    std::vector<Entry> images;
+    for (size_t i = 0; i < paths.size(); ++i)  // DICOMReader.cpp:148-152
+        images.push_back({&parsed[i], parsed[i].hasPatientID ? parsed[i].acquisitionTime : std::string(), paths[i]});
     std::stable_sort(images.begin(), images.end(), [](const Entry &a, const Entry &b) { return a.time < b.time; });
     std::vector<int> drop = indicesToRemove;
     std::sort(drop.begin(), drop.end(), std::greater<int>());
@@ -514,15 +598,14 @@ bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indices
     }
     if (images.empty())
         return fail("no DICOM images in " + dir);
-    // vtkExtractVOI of each image (clipped to it), appended along z
-    // (vtkImageAppend): every crop must cover the same x and y extent
+    // vtkExtractVOI of each image, appended along z (vtkImageAppend): every
+    // crop must cover the same x and y extent
     std::vector<int> zoff(images.size() + 1, 0);
+    std::vector<const Parsed *> files;
     int ext[4] = {0, 0, 0, 0};
     for (size_t i = 0; i < images.size(); ++i) {
-        const Frames &f = images[i].frames;
-        const int e[4] = {std::max(x0, 0), std::min(x0 + w - 1, f.columns - 1), std::max(y0, 0),
-                          std::min(y0 + h - 1, f.rows - 1)};
-        if (w <= 0 || h <= 0 || e[0] > e[1] || e[2] > e[3])
+        int e[4];
+        if (!crop_extent(*images[i].f, x0, y0, w, h, e))
             return fail("Empty data for raw projections " + images[i].path +
                         ": the detector region lies outside the image");
         if (i == 0)
@@ -531,27 +614,42 @@ bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indices
             return fail("image " + images[i].path + " differs in size from the first image");
         if (verbose)
             std::cout << "Ordered raw projections " << images[i].path << std::endl;
-        zoff[i + 1] = zoff[i] + f.frames;
+        zoff[i + 1] = zoff[i] + images[i].f->frames;
+        files.push_back(images[i].f);
     }
     auto img = ImageDataPtr::New();
     img->SetExtent(ext[0], ext[1], ext[2], ext[3], 0, zoff.back() - 1);
-    img->SetSpacing(images[0].frames.spacing[0], images[0].frames.spacing[1], 1.0);
+    img->SetSpacing(images[0].f->spacing[0], images[0].f->spacing[1], 1.0);
     img->AllocateScalars(VTK_FLOAT, 1);
-    float *dst = static_cast<float *>(img->GetScalarPointer());
-    const size_t nx = (size_t)(ext[1] - ext[0] + 1), ny = (size_t)(ext[3] - ext[2] + 1);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < (int)images.size(); ++i) {
-        const Frames &f = images[i].frames;
-        for (int z = 0; z < f.frames; ++z)
-            for (int y = ext[2]; y <= ext[3]; ++y) {
-                const float *src = f.values.data() + ((size_t)z * f.rows + (f.rows - 1 - y)) * f.columns + ext[0];
-                std::copy(src, src + nx, dst + (((size_t)zoff[i] + z) * ny + (size_t)(y - ext[2])) * nx);
-            }
-    }
+    decode_into(files, zoff, ext, static_cast<float *>(img->GetScalarPointer()));
     *out = img;
     return true;
 }
 }  // namespace kvdicom
+
+namespace {
+Result<ImageDataPtr> reading_problem(const std::string &what)
+{
+    std::cout << what << std::endl;
+    std::cout << "DataFileReadingProblem" << std::endl;
+    return DICOMReaderErrorCode::DataFileReadingProblem;
+}
+}  // namespace
+
+Result<ImageDataPtr> DICOMReader::Read(std::string const &p_dicomFilePath) const
+{
+    if (!std::filesystem::exists(p_dicomFilePath))
+        return reading_problem("Raw projections dicom file does not exist: " + p_dicomFilePath);
+    if (!m_tomoGeometry)
+        return reading_problem("DICOMReader: no geometry");
+    const Pixel bl = m_tomoGeometry->GetProjectionROIsBLPixelPositionOnDetector();
+    const Size2D size = m_tomoGeometry->projectionsRoisSize();
+    ImageDataPtr img;
+    std::string err;
+    if (!kvdicom::ReadImage(p_dicomFilePath, bl.x, bl.y, size.x, size.y, &img, &err))
+        return reading_problem(err);
+    return img;
+}
 
 Result<ImageDataPtr> DICOMReader::ReadDirectory(std::string const &p_dicomFilesContainedDirPath,
                                                 std::vector<int> const &p_indicesToRemove) const
@@ -591,14 +689,10 @@ extern "C" kvd_image *kvd_dicom_read(const char *path, int directory, const int3
         err[0] = '\0';
     // the detector region: {x, y, width, height}, or the whole image
     if (!directory) {
-        kvdicom::Frames f;
-        if (!kvdicom::CanReadFile(path))
-            return report(std::string("cannot read file: ") + path);
-        if (!kvdicom::ReadFile(path, &f, &msg))
-            return report(msg);
-        auto img = roi ? kvdicom::CropToImage(f, roi[0], roi[1], roi[2], roi[3], &msg)
-                       : kvdicom::CropToImage(f, 0, 0, f.columns, f.rows, &msg);
-        if (img == nullptr)
+        ImageDataPtr img;
+        const bool ok = roi ? kvdicom::ReadImage(path, roi[0], roi[1], roi[2], roi[3], &img, &msg)
+                            : kvdicom::ReadImage(path, 0, 0, 1 << 30, 1 << 30, &img, &msg);
+        if (!ok)
             return report(msg);
         return new kvd_image{img};
     }

@@ -57,6 +57,10 @@ bool ReadFile(const std::string &path, Frames *out, std::string *err);
 // (vtkExtractVOI, clipped to the image; extent kept).  Empty crop -> nullptr.
 ImageDataPtr CropToImage(const Frames &f, int x0, int y0, int w, int h, std::string *err);
 
+// One file cropped to x in [x0, x0 + w - 1], y in [y0, y0 + h - 1] (the
+// steps of DICOMReader::Read), decoded straight into the image.
+bool ReadImage(const std::string &path, int x0, int y0, int w, int h, ImageDataPtr *out, std::string *err);
+
 // The directory steps of DICOMReader::ReadDirectory with an explicit region.
 bool ReadDirectoryImages(const std::string &dir, const std::vector<int> &indicesToRemove, int x0, int y0, int w,
                          int h, bool verbose, ImageDataPtr *out, std::string *err);

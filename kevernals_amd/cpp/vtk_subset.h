@@ -73,19 +73,21 @@ public:
         for (int i = 0; i < 3; ++i)
             o[i] = origin_[i];
     }
-    // Only single-component float scalars are supported (VTK_FLOAT, 1).
+    // Only single-component float scalars are supported (VTK_FLOAT, 1).  As
+    // in VTK, the values are left uninitialised.
     void AllocateScalars(int type, int components)
     {
         (void)type;
         (void)components;
         const int *d = GetDimensions();
-        data_.assign((size_t)d[0] * (size_t)d[1] * (size_t)d[2], 0.f);
+        n_ = (size_t)d[0] * (size_t)d[1] * (size_t)d[2];
+        data_.reset(n_ ? new float[n_] : nullptr);
     }
-    void *GetScalarPointer() { return data_.empty() ? nullptr : data_.data(); }
+    void *GetScalarPointer() { return n_ ? data_.get() : nullptr; }
     void *GetScalarPointer(int x, int y, int z)
     {
         const int *d = GetDimensions();
-        return data_.data() + ((size_t)(z - extent_[4]) * d[1] + (size_t)(y - extent_[2])) * d[0] +
+        return data_.get() + ((size_t)(z - extent_[4]) * d[1] + (size_t)(y - extent_[2])) * d[0] +
                (size_t)(x - extent_[0]);
     }
     size_t GetNumberOfPoints()
@@ -100,7 +102,8 @@ private:
     int dims_[3] = {0, 0, 0};
     double spacing_[3] = {1.0, 1.0, 1.0};
     double origin_[3] = {0.0, 0.0, 0.0};
-    std::vector<float> data_;
+    std::unique_ptr<float[]> data_;
+    size_t n_ = 0;
 };
 
 template <class T>

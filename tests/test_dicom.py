@@ -42,7 +42,7 @@ def test_file_reads_to_known_values(name):
     if not c["ok"]:
         with pytest.raises(ValueError) as e:
             KIO.read_dicom(path)
-        assert c["why"] in str(e.value) or "cannot read" in str(e.value)
+        assert c["why"] in str(e.value) or "cannot read" in str(e.value).lower()
         return
     img = KIO.read_dicom(path)
     f = frames(name)
