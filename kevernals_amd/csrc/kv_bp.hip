@@ -224,8 +224,9 @@ struct BpShared {
 };
 
 // Backprojection of brick (bxi, byi, bzi) of depth zt <= ZT slices.  Called
-// by all 256 threads of a workgroup (it synchronises).
-template <int MODE, int EP, int ZT, bool UNIT>
+// by all 256 threads of a workgroup (it synchronises).  SR band loads per
+// thread are issued together before their LDS writes.
+template <int MODE, int EP, int ZT, bool UNIT, int SR>
 __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, const float *__restrict__ proj,
                                          float *__restrict__ vol, const float *__restrict__ sens, int bxi, int byi,
                                          int bzi, int zt, BpShared<ZT> &sm)
@@ -298,21 +299,49 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             __builtin_amdgcn_make_buffer_rsrc((void *)tx, (short)0, nx1 * g.nu * 4, 0x00020000);
         const __amdgpu_buffer_rsrc_t tys =
             __builtin_amdgcn_make_buffer_rsrc((void *)ty, (short)0, ny1 * g.nv * 4, 0x00020000);
-        for (int e = tid; e < (TX + 1) * XBW; e += 256) {
+        // a thread's band entries are staged in rounds of SR loads, each
+        // round issued before its LDS writes, so the staging waits for a few
+        // memory latencies instead of one per entry
+        constexpr int NX = (TX + 1) * XBW, NY = (TY + 1) * YBR;
+        const int ybase = ybase_s;
+        auto xload = [&](int e) {
             const int i = e / XBW, j = e - i * XBW;
             const unsigned k = (unsigned)(x0 + i), px = (unsigned)(xbase[i] + j);
-            xb[i * XBS + j] = __uint_as_float(
+            return __uint_as_float(
                 __builtin_amdgcn_raw_buffer_load_b32(txs, (int)((k * (unsigned)g.nu + px) * 4u), 0, 0));
+        };
+        auto yload = [&](int e) {
+            const int c = e / YBR, r = e - c * YBR;
+            const unsigned j = (unsigned)(y0 + c), py = (unsigned)(ybase + r);
+            return __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(tys, (int)((j * (unsigned)g.nv + py) * 4u), 0, 0));
+        };
+        for (int e0 = tid; e0 < NX; e0 += 256 * SR) {
+            float v[SR];
+#pragma unroll
+            for (int q = 0; q < SR; ++q)
+                v[q] = e0 + 256 * q < NX ? xload(e0 + 256 * q) : 0.f;
+#pragma unroll
+            for (int q = 0; q < SR; ++q) {
+                const int e = e0 + 256 * q;
+                if (e < NX)
+                    xb[(e / XBW) * XBS + e % XBW] = v[q];
+            }
         }
         const float *zsp = t.zs + ((size_t)p * nz1 + z0) * KV_ZS;
         for (int e = tid; e < (nzt + 1) * KV_ZS; e += 256)
             sm.zsb[e] = zsp[e];
-        const int ybase = ybase_s;
-        for (int e = tid; e < (TY + 1) * YBR; e += 256) {
-            const int c = e / YBR, r = e - c * YBR;
-            const unsigned j = (unsigned)(y0 + c), py = (unsigned)(ybase + r);
-            yb[e] = __uint_as_float(
-                __builtin_amdgcn_raw_buffer_load_b32(tys, (int)((j * (unsigned)g.nv + py) * 4u), 0, 0));
+        for (int e0 = tid; e0 < NY; e0 += 256 * SR) {
+            float v[SR];
+#pragma unroll
+            for (int q = 0; q < SR; ++q)
+                v[q] = e0 + 256 * q < NY ? yload(e0 + 256 * q) : 0.f;
+#pragma unroll
+            for (int q = 0; q < SR; ++q) {
+                const int e = e0 + 256 * q;
+                if (e < NY)
+                    yb[e] = v[q];
+            }
         }
         __syncthreads();
 
@@ -592,7 +621,10 @@ __global__ __launch_bounds__(256, 5) void bp_tiled_kernel(KvGeom g, KvTables t, 
                                                        float *__restrict__ vol, const float *__restrict__ sens)
 {
     __shared__ BpShared<ZT> sm;
-    bp_brick<MODE, EP, ZT, UNIT>(g, t, proj, vol, sens, blockIdx.x, blockIdx.y, blockIdx.z, ZT, sm);
+    // rounds of 3 loads: mode A measured 3 % faster; mode R's registers would
+    // spill with them
+    bp_brick<MODE, EP, ZT, UNIT, MODE == KV_MODE_A ? 3 : 1>(g, t, proj, vol, sens, blockIdx.x, blockIdx.y,
+                                                           blockIdx.z, ZT, sm);
 }
 
 // Persistent backprojection + MLEM update over a batch of reconstructions:
@@ -623,7 +655,7 @@ __global__ __launch_bounds__(256, 5) void bp_batch_kernel(const KvProblem *__res
         const int bz = (int)(local / per_z);
         const int rem = (int)(local - (long long)bz * per_z);
         const int by = rem / P.bp_bx, bx = rem - by * P.bp_bx;
-        bp_brick<MODE, KV_EP_MLEM, kZT, false>(P.g, P.t, P.ratio, P.vol, P.sens, bx, by, bz, P.bp_zt, sm);
+        bp_brick<MODE, KV_EP_MLEM, kZT, false, 1>(P.g, P.t, P.ratio, P.vol, P.sens, bx, by, bz, P.bp_zt, sm);
     }
 }
 
