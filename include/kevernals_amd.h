@@ -194,9 +194,9 @@ kv_status kv_batch_get_volume(kv_batch *b, int i, float *h_vol);
  * replays it.  *ms (optional) receives the device time of the run.
  * Synchronous. */
 kv_status kv_batch_mlem(kv_batch *b, const int32_t *iters, float relax, int use_graph, float *ms);
-/* Tile shapes of reconstruction i: forward layout 1, 2 or 4 (detector rows
- * per wavefront), backprojection brick depth 4 or 8 slices.  Results do not
- * depend on them. */
+/* Tile shapes of reconstruction i: forward layout 1, 2, 4 or 8 (detector
+ * rows per wavefront), backprojection brick depth 4 or 8 slices.  Results do
+ * not depend on them. */
 kv_status kv_batch_set_tiles(kv_batch *b, int i, int fp_layout, int bp_slices);
 /* Reconstruction i's context (owned by the batch: do not destroy it; its
  * geometry and tables are those of the batch) and device buffers, for

@@ -279,7 +279,7 @@ kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, int mode,
         P.ratio = c->d_ratio;
         P.sens = mode == KV_MODE_A ? c->d_sens : nullptr;
         P.iters = 0;
-        P.fp_layout = 1;
+        P.fp_layout = mode == KV_MODE_A ? KV_FP_BATCH_LAYOUT_A : KV_FP_BATCH_LAYOUT_R;
         P.bp_zt = 8;
     }
     if (st == KV_OK) {
@@ -401,7 +401,8 @@ kv_status kv_batch_set_tiles(kv_batch *b, int i, int fp_layout, int bp_slices)
 {
     if (kv_status st = check_index(b, i))
         return st;
-    if (!(fp_layout == 1 || fp_layout == 2 || fp_layout == 4) || !(bp_slices == 4 || bp_slices == 8)) {
+    if (!(fp_layout == 1 || fp_layout == 2 || fp_layout == 4 || fp_layout == 8) ||
+        !(bp_slices == 4 || bp_slices == 8)) {
         kv_set_error("unsupported tile shape (layout %d, slices %d)", fp_layout, bp_slices);
         return KV_ERR_INVALID;
     }
@@ -444,7 +445,7 @@ kv_status kv_batch_autotune(kv_batch *b, const char *cache_path)
             float t;
             if (!(ls >> d >> c >> l >> z >> t))
                 continue;
-            if (d == devkey && c >= 0 && c < 3 && (l == 1 || l == 2 || l == 4) && (z == 4 || z == 8)) {
+            if (d == devkey && c >= 0 && c < 3 && (l == 1 || l == 2 || l == 4 || l == 8) && (z == 4 || z == 8)) {
                 best_l[c] = l;
                 best_z[c] = z;
                 best_ms[c] = t;
@@ -470,7 +471,7 @@ kv_status kv_batch_autotune(kv_batch *b, const char *cache_path)
             if (e == hipSuccess)
                 e = hipMemcpyAsync(backup[(size_t)i], b->vol[(size_t)i], bytes, hipMemcpyDeviceToDevice, b->stream);
         }
-        const int layouts[3] = {1, 2, 4}, slices[2] = {4, 8};
+        const int layouts[4] = {1, 2, 4, 8}, slices[2] = {4, 8};
         std::vector<int32_t> it((size_t)b->n);
         for (int c = 0; c < 3 && e == hipSuccess && st == KV_OK; ++c) {
             if (!present[c] || best_l[c] != 0)

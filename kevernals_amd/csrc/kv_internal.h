@@ -61,7 +61,7 @@ struct KvProblem {
     float *ratio;
     const float *sens;         // mode A: sensitivity A^T 1
     int iters;                 // MLEM iterations for this reconstruction
-    int fp_layout;             // forward tile layout: 1, 2 or 4 rows per wavefront
+    int fp_layout;             // forward tile layout: 1, 2, 4 or 8 rows per wavefront
     int fp_tx, fp_ty;          // forward tiles per view
     long long fp_first;
     int bp_zt;                 // backprojection brick depth: 4 or 8 slices
@@ -121,6 +121,22 @@ inline int64_t kv_nvox(const KvGeom &g) { return (int64_t)g.nx * g.ny * g.nz; }
 inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 
 // ---- launchers (kv_fp.hip, kv_bp.hip, kv_misc.hip) ----------------------
+// Forward tiling (fp_tile_pixel: rows per wavefront; fp_xcd_tile: detector
+// columns per XCD stripe).  Measured at 1024x1024x64 with a 2048^2 detector
+// and 25 views: 8x8 rays per wavefront in 32-column stripes, mode R forward
+// 10.9 -> 8.8 ms and mode A 10.0 -> 8.3 ms against 64x1 rays per wavefront
+// round-robin.  The persistent batch kernels claim tiles in work-list order
+// (round-robin over XCDs); their default layouts are the fastest measured in
+// that order (mode R 8, mode A 4), and the autotuner may choose others.
+// The multi-volume likelihood walk (two or three gathers per step) keeps
+// 64x1 rays per wavefront, where it measured 14.9 / 24.3 ms for two / three
+// volumes against 16.3 / 31.5 ms in the 8x8 layout.
+constexpr int KV_FP_LAYOUT = 8;
+constexpr int KV_FP_LAYOUT_MULTI = 1;
+constexpr int KV_FP_STRIPE = 32;
+constexpr int KV_FP_BATCH_LAYOUT_R = 8;
+constexpr int KV_FP_BATCH_LAYOUT_A = 4;
+
 hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epilogue, const float *data,
                           float relax, hipStream_t s);
 // Mode-A forward pass.  KV_EP_MLEM writes the MLEM ratio data/Ax

@@ -29,7 +29,7 @@ def K():
 
 def problems():
     """Uneven sizes, none a multiple of the tile shapes: partial forward
-    tiles (64x4, 32x8, 16x16) and partial bricks (64x4x4, 64x4x8)."""
+    tiles (64x4, 32x8, 16x16, 8x32) and partial bricks (64x4x4, 64x4x8)."""
     from kevernals_amd import geometry as G
 
     specs = [
@@ -107,7 +107,7 @@ def test_batch_uneven_iterations_bit_exact(K, setup, use_graph):
 def test_batch_tile_shapes_do_not_change_results(K, setup):
     gs, datas, x0s = setup
     refs = [unbatched(K, g, datas[i], x0s[i], 2) for i, g in enumerate(gs)]
-    for layout in (1, 2, 4):
+    for layout in (1, 2, 4, 8):
         for slices in (4, 8):
             b = fresh_batch(K, setup)
             for i in range(len(gs)):
@@ -148,7 +148,7 @@ def test_batch_autotune_keeps_volumes_and_caches(K, setup, tmp_path):
     b = fresh_batch(K, setup)
     cache = str(tmp_path / "tiles.txt")
     tiles = b.autotune(cache)
-    assert all(l in (1, 2, 4) and z in (4, 8) for l, z in tiles)
+    assert all(l in (1, 2, 4, 8) and z in (4, 8) for l, z in tiles)
     for i in range(len(gs)):
         np.testing.assert_array_equal(b.get_volume(i), x0s[i])
     lines = open(cache).read().split()

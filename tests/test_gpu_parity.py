@@ -169,7 +169,8 @@ def test_fp_r_early_exit_quirk(K, O):
 
 
 def test_tiny_geometries_edge_cases(K, O):
-    """Odd sizes (not multiples of the 64x4 tiles), off-centre detector."""
+    """Odd sizes (not multiples of the forward tiles or backprojection
+    bricks), off-centre detector."""
     for vol, det in (((6, 5, 4), (12, 10)), ((7, 9, 3), (13, 5)), ((1, 1, 1), (3, 3))):
         g = tiny_geometry(nviews=3, vol=vol, det=det)
         o = O.Oracle(g)
