@@ -7,7 +7,8 @@
 // forward tiles and backprojection bricks sit in the batch-wide work list.
 // One iteration of the whole batch is two launches: the forward pass (ratio
 // epilogue) for every reconstruction, then the backprojection (update
-// epilogue).  Work items are claimed from a counter per launch; the counters
+// epilogue).  Work items are claimed from counters (one per XCD list for the
+// forward pass, one for the backprojection); the counters
 // are cleared by a small kernel (not hipMemsetAsync, which misbehaves inside
 // captured graphs on replay), so the same sequence runs eagerly or replayed.
 #include <algorithm>
@@ -54,11 +55,12 @@ void layout_items(kv_batch *b)
 {
     long long fp = 0, bp = 0;
     for (auto &P : b->probs) {
-        const int L = P.fp_layout;
-        P.fp_tx = (P.g.nu + 64 / L - 1) / (64 / L);
+        const int L = P.fp_layout, wu = 64 / L, span = KV_XCDS * std::max(1, KV_FP_STRIPE / wu);
+        P.fp_chunk = span / KV_XCDS;
+        P.fp_tx = ((P.g.nu + wu - 1) / wu + span - 1) / span * span;
         P.fp_ty = (P.g.nv + 4 * L - 1) / (4 * L);
         P.fp_first = fp;
-        fp += (long long)P.g.np * P.fp_tx * P.fp_ty;
+        fp += (long long)P.g.np * P.fp_ty * (P.fp_tx / KV_XCDS);
         P.bp_bx = (P.g.nx + 63) / 64;
         P.bp_by = (P.g.ny + 3) / 4;
         P.bp_bz = (P.g.nz + P.bp_zt - 1) / P.bp_zt;
@@ -76,9 +78,12 @@ kv_status upload_probs(kv_batch *b)
     return e == hipSuccess ? KV_OK : fail_hip(e, "upload of batch descriptors");
 }
 
+// per iteration: KV_XCDS forward-list counters, one backprojection counter
+constexpr int kCountersPerIter = KV_XCDS + 1;
+
 kv_status ensure_counters(kv_batch *b, int iters)
 {
-    const int need = 2 * std::max(iters, 1);
+    const int need = kCountersPerIter * std::max(iters, 1);
     if (need <= b->counters_cap)
         return KV_OK;
     if (b->d_counters)
@@ -101,13 +106,12 @@ kv_status ensure_counters(kv_batch *b, int iters)
 // The iteration sequence on the batch stream (eagerly or under capture).
 hipError_t enqueue_iterations(kv_batch *b, int iters, float relax)
 {
-    hipError_t e = kv_launch_zero_u64(b->d_counters, 2 * iters, b->stream);
+    hipError_t e = kv_launch_zero_u64(b->d_counters, kCountersPerIter * iters, b->stream);
     for (int k = 0; k < iters && e == hipSuccess; ++k) {
-        e = kv_launch_fp_batch(b->mode, b->d_probs, b->n, b->fp_total, b->d_counters + 2 * k, k, relax,
-                               b->fp_grid, b->stream);
+        unsigned long long *c = b->d_counters + (size_t)kCountersPerIter * k;
+        e = kv_launch_fp_batch(b->mode, b->d_probs, b->n, b->fp_total, c, k, relax, b->fp_grid, b->stream);
         if (e == hipSuccess)
-            e = kv_launch_bp_batch(b->mode, b->d_probs, b->n, b->bp_total, b->d_counters + 2 * k + 1, k,
-                                   b->bp_grid, b->stream);
+            e = kv_launch_bp_batch(b->mode, b->d_probs, b->n, b->bp_total, c + KV_XCDS, k, b->bp_grid, b->stream);
     }
     return e;
 }

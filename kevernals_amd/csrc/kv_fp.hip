@@ -337,39 +337,58 @@ __global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__rest
     fp_r_ray<EP>(g, vol, out, data, relax, p, u, v);
 }
 
-// Persistent forward pass over a batch of reconstructions: workgroups claim
-// tiles from a global counter until the batch's tile list is exhausted.
-// Problems whose iteration count is reached skip their tiles.
-template <int EP>
-__global__ __launch_bounds__(256) void fp_r_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
-                                                         long long total, unsigned long long *counter, int iteration,
-                                                         float relax)
+// Persistent forward pass over a batch of reconstructions.  The tiles sit in
+// KV_XCDS lists (KvProblem), list k holding the stripes an XCD-ordered
+// launch gives XCD k (fp_xcd_tile); `total` items per list, one counter per
+// list.  A workgroup claims tiles from the list of the XCD a one-dimensional
+// launch places it on (blockIdx.x % KV_XCDS), then from the other lists in
+// turn, until every list is exhausted: every tile is processed once whatever
+// the placement, which only decides the caches' locality.  Problems whose
+// iteration count is reached skip their tiles; padding columns (u past the
+// detector) are skipped at the bounds test.
+template <typename Ray>
+__device__ __forceinline__ void fp_batch_tiles(const KvProblem *__restrict__ probs, int nprob, long long total,
+                                               unsigned long long *counters, int iteration, Ray &&ray)
 {
     __shared__ long long item_s;
-    for (;;) {
-        if (threadIdx.x == 0)
-            item_s = (long long)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const long long item = item_s;
-        __syncthreads();
-        if (item >= total)
-            break;
-        int pi = 0;
-        while (pi + 1 < nprob && probs[pi + 1].fp_first <= item)
-            ++pi;
-        const KvProblem &P = probs[pi];
-        if (iteration >= P.iters)
-            continue;
-        const long long local = item - P.fp_first;
-        const int per_view = P.fp_tx * P.fp_ty;
-        const int p = (int)(local / per_view);
-        const int rem = (int)(local - (long long)p * per_view);
-        const int ty = rem / P.fp_tx, tx = rem - ty * P.fp_tx;
-        int u, v;
-        fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
-        if (u < P.g.nu && v < P.g.nv)
-            fp_r_ray<EP>(P.g, P.vol, P.ratio, P.data, relax, p, u, v);
+    const int own = (int)(blockIdx.x % (unsigned)KV_XCDS);
+    for (int c = 0; c < KV_XCDS; ++c) {
+        const int k = (own + c) % KV_XCDS;
+        for (;;) {
+            if (threadIdx.x == 0)
+                item_s = (long long)atomicAdd(counters + k, 1ull);
+            __syncthreads();
+            const long long item = item_s;
+            __syncthreads();
+            if (item >= total)
+                break;
+            int pi = 0;
+            while (pi + 1 < nprob && probs[pi + 1].fp_first <= item)
+                ++pi;
+            const KvProblem &P = probs[pi];
+            if (iteration >= P.iters)
+                continue;
+            const long long local = item - P.fp_first;
+            const long long ch = P.fp_chunk, nch = P.fp_tx / (KV_XCDS * P.fp_chunk);
+            const long long jc = local / ch, r = jc / nch;
+            const int tx = (int)(((jc - r * nch) * KV_XCDS + k) * ch + (local - jc * ch));
+            const int ty = (int)(r % P.fp_ty), p = (int)(r / P.fp_ty);
+            int u, v;
+            fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
+            if (u < P.g.nu && v < P.g.nv)
+                ray(P, p, u, v);
+        }
     }
+}
+
+template <int EP>
+__global__ __launch_bounds__(256) void fp_r_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
+                                                         long long total, unsigned long long *counters, int iteration,
+                                                         float relax)
+{
+    fp_batch_tiles(probs, nprob, total, counters, iteration, [&](const KvProblem &P, int p, int u, int v) {
+        fp_r_ray<EP>(P.g, P.vol, P.ratio, P.data, relax, p, u, v);
+    });
 }
 
 // ---------------------------------------------------------------- mode A --
@@ -618,33 +637,11 @@ __global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, in
 // Persistent mode-A forward pass (MLEM ratio) over a batch, as
 // fp_r_batch_kernel.
 __global__ __launch_bounds__(256) void fp_a_batch_kernel(const KvProblem *__restrict__ probs, int nprob,
-                                                         long long total, unsigned long long *counter, int iteration)
+                                                         long long total, unsigned long long *counters, int iteration)
 {
-    __shared__ long long item_s;
-    for (;;) {
-        if (threadIdx.x == 0)
-            item_s = (long long)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const long long item = item_s;
-        __syncthreads();
-        if (item >= total)
-            break;
-        int pi = 0;
-        while (pi + 1 < nprob && probs[pi + 1].fp_first <= item)
-            ++pi;
-        const KvProblem &P = probs[pi];
-        if (iteration >= P.iters)
-            continue;
-        const long long local = item - P.fp_first;
-        const int per_view = P.fp_tx * P.fp_ty;
-        const int p = (int)(local / per_view);
-        const int rem = (int)(local - (long long)p * per_view);
-        const int ty = rem / P.fp_tx, tx = rem - ty * P.fp_tx;
-        int u, v;
-        fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
-        if (u < P.g.nu && v < P.g.nv)
-            fp_a_ray<KV_EP_MLEM>(P.g, P.t, P.vol, P.ratio, P.data, p, u, v);
-    }
+    fp_batch_tiles(probs, nprob, total, counters, iteration, [&](const KvProblem &P, int p, int u, int v) {
+        fp_a_ray<KV_EP_MLEM>(P.g, P.t, P.vol, P.ratio, P.data, p, u, v);
+    });
 }
 
 // Records the accepted pairs of sampled rays through the same walks.
@@ -748,12 +745,12 @@ hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *
 }
 
 hipError_t kv_launch_fp_batch(int mode, const KvProblem *probs, int nprob, long long total,
-                              unsigned long long *counter, int iteration, float relax, int grid, hipStream_t s)
+                              unsigned long long *counters, int iteration, float relax, int grid, hipStream_t s)
 {
     if (mode == KV_MODE_A)
-        fp_a_batch_kernel<<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration);
+        fp_a_batch_kernel<<<grid, 256, 0, s>>>(probs, nprob, total, counters, iteration);
     else
-        fp_r_batch_kernel<KV_EP_MLEM><<<grid, 256, 0, s>>>(probs, nprob, total, counter, iteration, relax);
+        fp_r_batch_kernel<KV_EP_MLEM><<<grid, 256, 0, s>>>(probs, nprob, total, counters, iteration, relax);
     return hipGetLastError();
 }
 

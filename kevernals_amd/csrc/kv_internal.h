@@ -51,8 +51,11 @@ enum KvEpilogue { KV_EP_PLAIN = 0, KV_EP_MLEM = 1, KV_EP_SART = 2, KV_EP_COUNT =
 
 // One reconstruction of a batch, as the persistent kernels read it (device
 // memory, wave-uniform loads).  Work items are numbered consecutively over
-// the batch: forward tiles [fp_first, fp_first + views*fp_tx*fp_ty) and
-// backprojection bricks [bp_first, bp_first + bp_bx*bp_by*bp_bz).
+// the batch.  Forward tiles sit in KV_XCDS lists, one per XCD (stripes of
+// fp_chunk tile columns, stripe i in list i % KV_XCDS, as fp_xcd_tile): this
+// reconstruction holds items [fp_first, fp_first + views*fp_ty*fp_tx/KV_XCDS)
+// of every list, fp_tx being padded to whole stripes for all lists.
+// Backprojection bricks: [bp_first, bp_first + bp_bx*bp_by*bp_bz).
 struct KvProblem {
     KvGeom g;
     KvTables t;
@@ -62,7 +65,8 @@ struct KvProblem {
     const float *sens;         // mode A: sensitivity A^T 1
     int iters;                 // MLEM iterations for this reconstruction
     int fp_layout;             // forward tile layout: 1, 2, 4 or 8 rows per wavefront
-    int fp_tx, fp_ty;          // forward tiles per view
+    int fp_tx, fp_ty;          // forward tiles per view (fp_tx padded to a multiple of KV_XCDS * fp_chunk)
+    int fp_chunk;              // tile columns per XCD stripe
     long long fp_first;
     int bp_zt;                 // backprojection brick depth: 4 or 8 slices
     int bp_bx, bp_by, bp_bz;   // bricks per axis
@@ -121,16 +125,18 @@ inline int64_t kv_nvox(const KvGeom &g) { return (int64_t)g.nx * g.ny * g.nz; }
 inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 
 // ---- launchers (kv_fp.hip, kv_bp.hip, kv_misc.hip) ----------------------
-// Forward tiling (fp_tile_pixel: rows per wavefront; fp_xcd_tile: detector
-// columns per XCD stripe).  Measured at 1024x1024x64 with a 2048^2 detector
-// and 25 views: 8x8 rays per wavefront in 32-column stripes, mode R forward
-// 10.9 -> 8.8 ms and mode A 10.0 -> 8.3 ms against 64x1 rays per wavefront
-// round-robin.  The persistent batch kernels claim tiles in work-list order
-// (round-robin over XCDs); their default layouts are the fastest measured in
-// that order (mode R 8, mode A 4), and the autotuner may choose others.
+// Forward tiling (fp_tile_pixel: rows per wavefront; fp_xcd_tile and the
+// batch kernels' per-XCD lists: detector columns per XCD stripe).  Measured
+// at 1024x1024x64 with a 2048^2 detector and 25 views: 8x8 rays per
+// wavefront in 32-column stripes, mode R forward 10.9 -> 8.8 ms and mode A
+// 10.0 -> 8.3 ms against 64x1 rays per wavefront round-robin.  The batch
+// kernels' default layouts: mode R 8, mode A 4 (eight full-size mode-A
+// reconstructions: 3.08 s to 1 % RMSE against 3.12 s in 8x8); the autotuner
+// may choose others.
 // The multi-volume likelihood walk (two or three gathers per step) keeps
 // 64x1 rays per wavefront, where it measured 14.9 / 24.3 ms for two / three
 // volumes against 16.3 / 31.5 ms in the 8x8 layout.
+constexpr int KV_XCDS = 8;
 constexpr int KV_FP_LAYOUT = 8;
 constexpr int KV_FP_LAYOUT_MULTI = 1;
 constexpr int KV_FP_STRIPE = 32;
@@ -160,9 +166,11 @@ hipError_t kv_launch_bp_a(const KvGeom &g, const KvTables &t, const float *proj,
 // out = proj * length factor of each ray (mode-A tables), np*nv*nu values.
 hipError_t kv_launch_scale_a(const KvGeom &g, const KvTables &t, const float *proj, float *out, hipStream_t s);
 // Batched persistent passes (MLEM epilogues, mode R or A).  `grid` workgroups
-// of 256 threads claim work items from *counter (zero before the launch).
+// of 256 threads claim work items: the forward pass from counters[0 ..
+// KV_XCDS-1] (one per XCD list of `total` items), the backprojection from
+// *counter; all zero before the launch.
 hipError_t kv_launch_fp_batch(int mode, const KvProblem *probs, int nprob, long long total,
-                              unsigned long long *counter, int iteration, float relax, int grid, hipStream_t s);
+                              unsigned long long *counters, int iteration, float relax, int grid, hipStream_t s);
 hipError_t kv_launch_bp_batch(int mode, const KvProblem *probs, int nprob, long long total,
                               unsigned long long *counter, int iteration, int grid, hipStream_t s);
 hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
