@@ -610,11 +610,11 @@ struct KvMulti {
 };
 
 template <int NV>
-__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, int gxp, int gy, int chunk)
+__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, int layout, int gxp, int gy, int chunk)
 {
     int tx, ty, p, u, v;
     fp_xcd_tile(gxp, gy, chunk, tx, ty, p);
-    fp_tile_pixel(KV_FP_LAYOUT_MULTI, tx, ty, threadIdx.x, u, v);
+    fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
     RayA r;
@@ -713,11 +713,12 @@ hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols
     KvMulti m{};
     for (int i = 0; i < n; ++i)
         m.vol[i] = vols[i], m.out[i] = outs[i];
-    const FpGrid fg(g, KV_FP_LAYOUT_MULTI);
+    const int L = KV_FP_LAYOUT_MULTI[std::min(std::max(n, 1), KV_LL_MAX) - 1];
+    const FpGrid fg(g, L);
     switch (n) {
-    case 1: fp_a_multi_kernel<1><<<fg.blocks, 256, 0, s>>>(g, m, fg.gxp, fg.gy, fg.chunk); break;
-    case 2: fp_a_multi_kernel<2><<<fg.blocks, 256, 0, s>>>(g, m, fg.gxp, fg.gy, fg.chunk); break;
-    default: fp_a_multi_kernel<3><<<fg.blocks, 256, 0, s>>>(g, m, fg.gxp, fg.gy, fg.chunk); break;
+    case 1: fp_a_multi_kernel<1><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
+    case 2: fp_a_multi_kernel<2><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
+    default: fp_a_multi_kernel<3><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
     }
     return hipGetLastError();
 }

@@ -133,12 +133,13 @@ inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 // kernels' default layouts: mode R 8, mode A 4 (eight full-size mode-A
 // reconstructions: 3.08 s to 1 % RMSE against 3.12 s in 8x8); the autotuner
 // may choose others.
-// The multi-volume likelihood walk (two or three gathers per step) keeps
-// 64x1 rays per wavefront, where it measured 14.9 / 24.3 ms for two / three
-// volumes against 16.3 / 31.5 ms in the 8x8 layout.
+// The multi-volume likelihood walk takes its layout by volume count (one,
+// two, three gathers per step): 8.5 / 14.1 / 21.7 ms in 8x8 / 16x4 / 32x2
+// rays per wavefront, against 10.4 / 15.7 / 25.2 ms in 64x1 and 8.5 / 17.9 /
+// 31.0 ms in 8x8 for all counts.
 constexpr int KV_XCDS = 8;
 constexpr int KV_FP_LAYOUT = 8;
-constexpr int KV_FP_LAYOUT_MULTI = 1;
+constexpr int KV_FP_LAYOUT_MULTI[KV_LL_MAX] = {8, 4, 2};
 constexpr int KV_FP_STRIPE = 32;
 constexpr int KV_FP_BATCH_LAYOUT_R = 8;
 constexpr int KV_FP_BATCH_LAYOUT_A = 4;
