@@ -315,22 +315,22 @@ __device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int ti
 // outer bands idle: 11.4 ms against 8.8 ms).  gxp = tile columns rounded up
 // to a multiple of 8 x chunk; tiles of padding columns (u past the detector)
 // exit at the kernel's bounds test.
-__device__ __forceinline__ void fp_xcd_tile(int gxp, int gy, int chunk, int &tx, int &ty, int &p)
+__device__ __forceinline__ void fp_xcd_tile(int gxp, int gy, int chunk, int p0, int &tx, int &ty, int &p)
 {
     const unsigned w = blockIdx.x, c = (unsigned)chunk, nch = (unsigned)gxp / (8u * c);
     const unsigned j = w / 8u, t = j % c, jc = j / c, ci = jc % nch, r = jc / nch;
     tx = (int)((ci * 8u + w % 8u) * c + t);
     ty = (int)(r % (unsigned)gy);
-    p = (int)(r / (unsigned)gy);
+    p = p0 + (int)(r / (unsigned)gy);
 }
 
 template <int EP>
 __global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
                                                    const float *__restrict__ data, float relax, int layout,
-                                                   int gxp, int gy, int chunk)
+                                                   int gxp, int gy, int chunk, int p0)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, tx, ty, p);
+    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
     fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
@@ -591,10 +591,10 @@ __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const KvTables &t, con
 template <int EP>
 __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, KvTables t, const float *__restrict__ vol,
                                                    float *__restrict__ out, const float *__restrict__ data, int layout,
-                                                   int gxp, int gy, int chunk)
+                                                   int gxp, int gy, int chunk, int p0)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, tx, ty, p);
+    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
     fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
@@ -610,10 +610,10 @@ struct KvMulti {
 };
 
 template <int NV>
-__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, int layout, int gxp, int gy, int chunk)
+__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, int layout, int gxp, int gy, int chunk, int p0)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, tx, ty, p);
+    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
     fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
@@ -678,18 +678,35 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
 
 }  // namespace
 
-// One-dimensional launch of the single-reconstruction forward kernels over
-// gxp x gy tiles per view (fp_xcd_tile).
+// One-dimensional launches of the single-reconstruction forward kernels over
+// gxp x gy tiles per view (fp_xcd_tile), as many views per launch as keep a
+// launch within 2^32 threads.
 struct FpGrid {
-    int gxp, gy, chunk;
-    unsigned blocks;
+    int gxp, gy, chunk, views;
     FpGrid(const KvGeom &g, int layout)
     {
         const int wu = 64 / layout, wv = 4 * layout;
         chunk = std::max(1, KV_FP_STRIPE / wu);
         gxp = ((g.nu + wu - 1) / wu + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
         gy = (g.nv + wv - 1) / wv;
-        blocks = (unsigned)gxp * (unsigned)gy * (unsigned)g.np;
+        const unsigned long long per_view = (unsigned long long)gxp * (unsigned long long)gy * 256ull;
+        views = (int)std::min<unsigned long long>((unsigned long long)g.np, 0xFFFFFFFFull / per_view);
+    }
+    // Runs launch(blocks, p0) for each range of views; a view too large for
+    // one launch is an invalid configuration.
+    template <typename F>
+    hipError_t run(const KvGeom &g, F &&launch) const
+    {
+        if (views < 1)
+            return hipErrorInvalidConfiguration;
+        for (int p0 = 0; p0 < g.np; p0 += views) {
+            const int nv = std::min(views, g.np - p0);
+            launch((unsigned)gxp * (unsigned)gy * (unsigned)nv, p0);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
     }
 };
 
@@ -698,14 +715,22 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
 {
     const int L = KV_FP_LAYOUT;
     const FpGrid fg(g, L);
-    const unsigned nb = fg.blocks;
-    switch (epilogue) {
-    case KV_EP_PLAIN: fp_r_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk); break;
-    case KV_EP_MLEM: fp_r_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk); break;
-    case KV_EP_COUNT: fp_r_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk); break;
-    default: fp_r_kernel<KV_EP_SART><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk); break;
-    }
-    return hipGetLastError();
+    return fg.run(g, [&](unsigned nb, int p0) {
+        switch (epilogue) {
+        case KV_EP_PLAIN:
+            fp_r_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            break;
+        case KV_EP_MLEM:
+            fp_r_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            break;
+        case KV_EP_COUNT:
+            fp_r_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            break;
+        default:
+            fp_r_kernel<KV_EP_SART><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            break;
+        }
+    });
 }
 
 hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *const *outs, hipStream_t s)
@@ -715,12 +740,13 @@ hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols
         m.vol[i] = vols[i], m.out[i] = outs[i];
     const int L = KV_FP_LAYOUT_MULTI[std::min(std::max(n, 1), KV_LL_MAX) - 1];
     const FpGrid fg(g, L);
-    switch (n) {
-    case 1: fp_a_multi_kernel<1><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
-    case 2: fp_a_multi_kernel<2><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
-    default: fp_a_multi_kernel<3><<<fg.blocks, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk); break;
-    }
-    return hipGetLastError();
+    return fg.run(g, [&](unsigned nb, int p0) {
+        switch (n) {
+        case 1: fp_a_multi_kernel<1><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
+        case 2: fp_a_multi_kernel<2><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
+        default: fp_a_multi_kernel<3><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
+        }
+    });
 }
 
 hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, float *out, int epilogue,
@@ -728,14 +754,14 @@ hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, 
 {
     const int L = KV_FP_LAYOUT;
     const FpGrid fg(g, L);
-    const unsigned nb = fg.blocks;
-    if (epilogue == KV_EP_PLAIN)
-        fp_a_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk);
-    else if (epilogue == KV_EP_COUNT)
-        fp_a_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk);
-    else
-        fp_a_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk);
-    return hipGetLastError();
+    return fg.run(g, [&](unsigned nb, int p0) {
+        if (epilogue == KV_EP_PLAIN)
+            fp_a_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+        else if (epilogue == KV_EP_COUNT)
+            fp_a_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+        else
+            fp_a_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+    });
 }
 
 hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx, float *w,

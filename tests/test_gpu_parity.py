@@ -305,6 +305,39 @@ def test_log_likelihoods_share_one_walk_bit_for_bit(K, O):
 
 
 # ---------------------------------------------------------- shift-and-add --
+def test_forward_launches_split_over_views(K, O):
+    """A 1 x 4096 detector over 4097 identical views takes more than 2^32
+    threads in the forward kernels' one-dimensional, XCD-striped launch
+    order (tile columns padded to whole stripes), so the views are split over
+    two launches: every view must equal view 0, view 0 must equal the
+    oracle's, and the likelihood walk over several volumes must equal the
+    single-volume one."""
+    from kevernals_amd import geometry as G
+
+    g1 = G.arc_geometry((8, 8, 4), (2.0, 2.0, 4.0), (1, 4096), (16.0, 0.006), 1, 0.0)
+    nviews = 4097
+    g = g1.scaled(nviews=nviews, det=np.repeat(g1.det, nviews, axis=0), src=np.repeat(g1.src, nviews, axis=0))
+    o1 = O.Oracle(g1)
+    r = rng(31)
+    x = r.random(g.vol_shape).astype(np.float32) + np.float32(0.01)
+    ctx = K.Context(0, g)
+    out = torch.empty(g.proj_shape, device="cuda")
+    for mode, ref in ((K.MODE_R, o1.fp_r(x)), (K.MODE_A, o1.fp_a(x))):
+        ctx.forward_project(dev(x), out, mode)
+        got = host(out)
+        assert (got == got[:1]).all(), f"mode {mode}: views differ"
+        if mode == K.MODE_R:
+            np.testing.assert_array_equal(got[:1], ref)
+        else:
+            np.testing.assert_allclose(got[:1], ref, rtol=1e-5, atol=1e-6)
+    assert np.count_nonzero(got[0]) > 1000
+    d = dev(np.ascontiguousarray(got) * np.float32(1.1))
+    vols = [dev(x * np.float32(s)) for s in (1.0, 0.9, 1.2)]
+    single = [ctx.log_likelihood(d, v) for v in vols]
+    for n in (1, 2, 3):
+        assert ctx.log_likelihoods(d, vols[:n]) == single[:n]
+
+
 def test_shift_and_add_bit_exact(K, O):
     g = small()
     proj = rng(28).random(g.proj_shape).astype(np.float32)
