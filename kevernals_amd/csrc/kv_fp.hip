@@ -314,12 +314,13 @@ __device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int ti
 // the (uneven) ray lengths balanced (one contiguous band per XCD left the
 // outer bands idle: 11.4 ms against 8.8 ms).  gxp = tile columns rounded up
 // to a multiple of 8 x chunk; tiles of padding columns (u past the detector)
-// exit at the kernel's bounds test.
+// exit at the kernel's bounds test.  A launch covers views p0, p0 + 1, ...
 __device__ __forceinline__ void fp_xcd_tile(int gxp, int gy, int chunk, int p0, int &tx, int &ty, int &p)
 {
-    const unsigned w = blockIdx.x, c = (unsigned)chunk, nch = (unsigned)gxp / (8u * c);
-    const unsigned j = w / 8u, t = j % c, jc = j / c, ci = jc % nch, r = jc / nch;
-    tx = (int)((ci * 8u + w % 8u) * c + t);
+    constexpr unsigned X = KV_XCDS;
+    const unsigned w = blockIdx.x, c = (unsigned)chunk, nch = (unsigned)gxp / (X * c);
+    const unsigned j = w / X, t = j % c, jc = j / c, ci = jc % nch, r = jc / nch;
+    tx = (int)((ci * X + w % X) * c + t);
     ty = (int)(r % (unsigned)gy);
     p = p0 + (int)(r / (unsigned)gy);
 }
@@ -687,7 +688,7 @@ struct FpGrid {
     {
         const int wu = 64 / layout, wv = 4 * layout;
         chunk = std::max(1, KV_FP_STRIPE / wu);
-        gxp = ((g.nu + wu - 1) / wu + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
+        gxp = ((g.nu + wu - 1) / wu + KV_XCDS * chunk - 1) / (KV_XCDS * chunk) * (KV_XCDS * chunk);
         gy = (g.nv + wv - 1) / wv;
         const unsigned long long per_view = (unsigned long long)gxp * (unsigned long long)gy * 256ull;
         views = (int)std::min<unsigned long long>((unsigned long long)g.np, 0xFFFFFFFFull / per_view);
