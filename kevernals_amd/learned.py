@@ -10,7 +10,7 @@ x <- x / A^T 1 * A^T(data / Ax), the mode-A form of recons::MLEM
 (KevLoq/Kevernals Reconstructors.h:182-242).  The CNN maps an early MLEM
 estimate towards the 100-iteration one; a CNN step is kept only if it does
 not lower the Poisson log-likelihood sum(data*log(Ax) - Ax), so plain MLEM
-stays both the fallback and the judge.
+stays both the fallback and the arbiter.
 """
 from __future__ import annotations
 

@@ -5,7 +5,7 @@ network gains over plain MLEM.
   time, and the run is then bit-identical to plain MLEM (plain MLEM is the
   fallback).
 - Accepted candidates never lower the log-likelihood (plain MLEM is the
-  judge).
+  arbiter).
 - With the committed trained weights, on held-out phantoms of the training
   geometry (256x256x32, 15 views, I0 1e4), the accepted CNN step moves the
   estimate towards the 100-iteration MLEM result by more than four MLEM
