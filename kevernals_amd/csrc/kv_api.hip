@@ -51,6 +51,22 @@ kv_status dev_alloc(void **p, size_t bytes)
     return KV_OK;
 }
 
+// Grows the context's reduction buffer to at least n doubles.  The capacity
+// is cleared before the reallocation, so a failed allocation leaves a null
+// buffer of capacity 0 (the next call allocates again) rather than a null
+// buffer that a later, smaller request would launch into.
+kv_status ensure_red(kv_context *ctx, size_t n)
+{
+    if (ctx->red_cap >= n)
+        return KV_OK;
+    free_dev(ctx->d_red);
+    ctx->red_cap = 0;
+    if (kv_status st = dev_alloc((void **)&ctx->d_red, n * sizeof(double)))
+        return st;
+    ctx->red_cap = n;
+    return KV_OK;
+}
+
 kv_status check_ctx(kv_context *ctx, bool need_geom)
 {
     if (!ctx) {
@@ -468,12 +484,8 @@ kv_status kv_count_pairs(kv_context *ctx, int mode, int64_t *fp_pairs, int64_t *
     if (kv_status st = ensure_tables(ctx, mode))
         return st;
     const int nb = 1024;
-    if (ctx->red_cap < nb) {
-        free_dev(ctx->d_red);
-        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
-            return st;
-        ctx->red_cap = nb;
-    }
+    if (kv_status st = ensure_red(ctx, (size_t)nb))
+        return st;
     float *d_cnt = nullptr;
     if (kv_status st = dev_alloc((void **)&d_cnt, (size_t)kv_nvox(ctx->g) * 4))
         return st;
@@ -635,12 +647,8 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
         return KV_ERR_INVALID;
     }
     const int nb = 1024;
-    if (ctx->red_cap < nb) {
-        free_dev(ctx->d_red);
-        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
-            return st;
-        ctx->red_cap = nb;
-    }
+    if (kv_status st = ensure_red(ctx, (size_t)nb))
+        return st;
     KV_TRY_HIP(kv_launch_fp_a(ctx->g, ctx->tab_a, d_vol, ctx->d_ratio, KV_EP_PLAIN, nullptr, ctx->stream));
     KV_TRY_HIP(kv_launch_loglik(ctx->g, d_data, ctx->d_ratio, ctx->d_red, nb, ctx->stream));
     std::vector<double> part(nb);
@@ -668,12 +676,8 @@ kv_status kv_log_likelihoods(kv_context *ctx, const float *d_data, const float *
         }
     }
     const int nb = 1024;
-    if (ctx->red_cap < nb) {
-        free_dev(ctx->d_red);
-        if (kv_status st = dev_alloc((void **)&ctx->d_red, nb * sizeof(double)))
-            return st;
-        ctx->red_cap = nb;
-    }
+    if (kv_status st = ensure_red(ctx, (size_t)nb))
+        return st;
     const size_t nray = (size_t)kv_nray(ctx->g);
     if (n > 1 && !ctx->d_ll) {
         if (kv_status st = dev_alloc((void **)&ctx->d_ll, (KV_LL_MAX - 1) * nray * sizeof(float)))
@@ -794,12 +798,8 @@ kv_status kv_traj_gram(kv_context *ctx, const float *const *d_iterates, int h, d
     const int64_t n = kv_nvox(ctx->g);
     const int nb = kv_traj_gram_blocks(n), np = kv_traj_pairs(h);
     const int need = nb * np * 16;
-    if (ctx->red_cap < need) {
-        free_dev(ctx->d_red);
-        if (kv_status st = dev_alloc((void **)&ctx->d_red, (size_t)need * sizeof(double)))
-            return st;
-        ctx->red_cap = need;
-    }
+    if (kv_status st = ensure_red(ctx, (size_t)need))
+        return st;
     KV_TRY_HIP(kv_launch_traj_gram(tr, h, n, ctx->d_sens, ctx->d_red, ctx->stream));
     std::vector<double> part((size_t)need);
     KV_TRY_HIP(hipMemcpyAsync(part.data(), ctx->d_red, (size_t)need * sizeof(double), hipMemcpyDeviceToHost,

@@ -1,9 +1,13 @@
 // Forward projectors for gfx950.
 //
 // One ray per lane; a 256-thread workgroup is four wavefronts, each covering
-// 64 consecutive detector pixels of one row, so the 64 rays of a wavefront are
-// neighbours in u and their voxel gathers at each step fall on neighbouring
-// x addresses of the x-fastest volume.
+// a (64/L) x L tile of detector pixels (8 x 8 for the plain walk; 16 x 4 and
+// 32 x 2 for the walks that gather two and three volumes per step), so the
+// rays of a wavefront stay close in u and v and their voxel gathers at each
+// step fall on nearby lines of the x-fastest volume.  Workgroup tiles are
+// launched in a one-dimensional order that puts stripes of 32 detector
+// columns on the eight XCDs in turn (stripe i on XCD i % 8, each XCD walking
+// its stripes column-first), so every XCD's L2 holds one band of the volume.
 //
 // Mode R reproduces CudaPerformProjection (KevLoq/Kevernals
 // source/modules/reconstruction/Projector.cu:227-372) bit for bit.  The

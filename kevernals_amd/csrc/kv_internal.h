@@ -89,7 +89,7 @@ struct kv_context {
     float *d_sens = nullptr;   // mode-A sensitivity A^T 1
     bool sens_ok = false;
     double *d_red = nullptr;   // partial sums of deterministic reductions
-    int red_cap = 0;
+    size_t red_cap = 0;        // doubles in d_red
     float *d_ll = nullptr;     // projections of the further volumes of kv_log_likelihoods
 };
 

@@ -351,6 +351,33 @@ def test_shift_and_add_bit_exact(K, O):
     np.testing.assert_array_equal(host(out), ref)
 
 
+def test_shift_and_add_row_shifts_closed_form(K, O):
+    """The hand-derived landing rows of tests/test_oracle.py (impulse at row
+    20 of view p, column p, lands at row 20 - t_p; both signs of the source
+    y, both signs of mZ - mF, truncation toward zero) through the kernel."""
+    from test_oracle import SAA_CASES, saa_impulses
+
+    from kevernals_amd.geometry import Geometry
+
+    imp = saa_impulses()
+    P, nv, nu = imp.shape
+    for fz, z, ys, rows in SAA_CASES:
+        g = Geometry(4, 4, 2, 1.0, 0.5, 1.0, nu, nv, P, 1.0, 1.0, np.zeros((P, 3)),
+                     np.array([[0.0, y, 100.0] for y in ys]))
+        ctx = K.Context(0, g)
+        out = torch.full((2, nv, nu), -1.0, device="cuda")
+        ctx.shift_and_add(dev(imp), out, np.array(ys, np.float32), [z, fz], 100.0, fz, 0.5)
+        got = host(out)
+        expect = np.zeros((2, nv, nu), np.float32)
+        for p, r in enumerate(rows):
+            expect[0, r, p] = np.float32(0.2)
+        expect[1, 20, :] = np.float32(0.2)
+        np.testing.assert_array_equal(got, expect)
+        rc, ref = O.saa(imp, np.array(ys, np.float32), [z, fz], 100.0, fz, 0.5)
+        assert rc == 0
+        np.testing.assert_array_equal(got, ref)
+
+
 def test_shift_and_add_preconditions(K):
     g = small()
     ctx = K.Context(0, g)

@@ -369,18 +369,61 @@ def test_saa_closed_forms(oracle_mod):
     rc, out = oracle_mod.saa(proj, np.linspace(-20, 20, P), [zf, zf], sid, zf, vdy)
     assert rc == 0
     np.testing.assert_allclose(out[0], proj.mean(axis=0), rtol=1e-6)
-    # impulse: one pixel per view lands at v - t_p with weight 1/P
-    imp = np.zeros((P, nv, nu), np.float32)
-    imp[2, 6, 4] = 1.0
-    ys = np.array([-8.0, -4.0, 0.0, 4.0, 8.0], np.float32)
-    z = 30.0
-    rc, out = oracle_mod.saa(imp, ys, [z, 0.0], sid, 0.0, vdy)
-    mz = np.float32(sid) / (np.float32(sid) - np.float32(z))
-    mf = np.float32(sid) / (np.float32(sid) - np.float32(0.0))
-    t2 = int(np.float32(ys[2] * np.float32(mz - mf)) / np.float32(vdy))
-    assert rc == 0 and out[0, 6 - t2, 4] == np.float32(0.2) and np.count_nonzero(out[0]) == 1
     # preconditions (Reconstructors.h:275-305,319-323)
+    ys = np.array([-8.0, -4.0, 0.0, 4.0, 8.0], np.float32)
     assert oracle_mod.saa(proj, ys, [1.0], sid, 0.0, vdy)[0] == -1
     assert oracle_mod.saa(proj[:1], ys[:1], [1.0, 2.0], sid, 0.0, vdy)[0] == -2
     assert oracle_mod.saa(proj, ys, [1.0, 2.0], sid, sid, vdy)[0] == -3
     assert oracle_mod.saa(proj, ys, [1.0, sid], sid, 0.0, vdy)[0] == -4
+
+
+# Shift-and-add row shifts, hand-derived (Reconstructors.h:308-331,355-368):
+# the VOI of view p starts at row t_p = (int)(y_p * (mZ - mF) / dy) of the
+# zero-padded stack and is translated to row 0, so the pixel (r, c) of view p
+# lands at row r - t_p of the plane, with weight 1/P.  Each view carries one
+# impulse at row 20 in its own column c = p, so every view's landing row is
+# read separately.  The magnifications are exact in binary (sid 100 mm):
+#   fulcrum 0,  plane 50: mF = 1,    mZ = 2   -> coefficient +1
+#   fulcrum 20, plane 60: mF = 1.25, mZ = 2.5 -> coefficient +1.25
+#   fulcrum 20, plane 0:  mF = 1.25, mZ = 1   -> coefficient -0.25
+# and the source y values make the quotient non-integral for both signs, so a
+# floor in place of the truncation toward zero moves the negative ones.
+SAA_CASES = [
+    # fulcrum z, plane z, source y per view, expected landing rows (20 - t_p)
+    (0.0, 50.0, [-3.0, -1.3, 0.0, 1.3, 3.0], [26, 22, 20, 18, 14]),  # t = -6, -2, 0, 2, 6
+    (20.0, 60.0, [4.0, -4.0, 2.2, -2.2, 0.0], [10, 30, 15, 25, 20]),  # t = 10, -10, 5, -5, 0
+    (20.0, 0.0, [4.0, -4.0, 2.2, -2.2, 0.0], [22, 18, 21, 19, 20]),  # t = -2, 2, -1, 1, 0
+]
+
+
+def saa_impulses(nviews=5, nv=40, row=20):
+    imp = np.zeros((nviews, nv, nviews), np.float32)
+    for p in range(nviews):
+        imp[p, row, p] = 1.0
+    return imp
+
+
+@pytest.mark.parametrize("fz,z,ys,rows", SAA_CASES)
+def test_saa_row_shift_closed_form(oracle_mod, fz, z, ys, rows):
+    imp = saa_impulses()
+    rc, out = oracle_mod.saa(imp, np.array(ys, np.float32), [z, fz], 100.0, fz, 0.5)
+    assert rc == 0
+    expect = np.zeros(out.shape[1:], np.float32)
+    for p, r in enumerate(rows):
+        expect[r, p] = np.float32(0.2)
+    np.testing.assert_array_equal(out[0], expect)
+    # the second plane sits at the fulcrum: no shift, every impulse on row 20
+    expect = np.zeros(out.shape[1:], np.float32)
+    expect[20, :] = np.float32(0.2)
+    np.testing.assert_array_equal(out[1], expect)
+
+
+def test_saa_shift_out_of_range_is_zero_padded(oracle_mod):
+    # t = +-10 rows on a 12-row stack: the impulse at row 5 leaves the image
+    imp = np.zeros((2, 12, 2), np.float32)
+    imp[0, 5, 0] = 1.0
+    imp[1, 5, 1] = 1.0
+    rc, out = oracle_mod.saa(imp, np.array([4.0, -4.0], np.float32), [60.0, 20.0], 100.0, 20.0, 0.5)
+    assert rc == 0
+    assert not out[0].any()
+    assert out[1, 5, 0] == np.float32(0.5) and out[1, 5, 1] == np.float32(0.5)
