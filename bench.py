@@ -55,15 +55,20 @@ def make_data(K, ctx, g, torch, seed=2):
     return mu, data
 
 
-def cpu_baseline(g, mu_h, data_h, seconds):
-    """Oracle FP+BP (mode R, OpenMP over rays / voxels) on a sample spread
-    over all views and slices, extrapolated to one full forward+back pass."""
-    from oracle.oracle import Oracle
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
-    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = Oracle(g, nt)
-    rng = np.random.Generator(np.random.PCG64(7))
-    # calibrate with a small sample, then size the sample to ~seconds
+
+def _cpu_sample(o, g, mu_h, data_h, seconds, rng):
+    """Oracle FP (whole detector rows, every view) and BP (random voxels),
+    sized to about `seconds`, extrapolated to one full forward+back pass."""
     rows_per_view = 2
     t = time.time()
     for p in range(g.nviews):
@@ -91,15 +96,62 @@ def cpu_baseline(g, mu_h, data_h, seconds):
     o.bp_r_voxels(data_h, vox)
     bp_s = time.time() - t
     full_s = fp_s / rays_done * g.nrays + bp_s / n_vox * g.nvox
+    return full_s, rays_done, n_rows, n_vox, fp_s + bp_s
+
+
+def cpu_small_full(nt):
+    """The 128x128x32 workload run in full by the oracle (mode R, OpenMP):
+    forward projection, backprojection and 10 MLEM iterations, as the
+    reference's host loops compose them (Reconstructors.h:182-242)."""
+    from kevernals_amd import geometry as G
+    from kevernals_amd.phantom import phantom
+    from oracle.oracle import Oracle
+
+    g = G.small_plain()
+    o = Oracle(g, nt)
+    mu = phantom(g, 1)
+    t = time.time()
+    fp = o.fp_r(mu)
+    fp_s = time.time() - t
+    t = time.time()
+    o.bp_r(fp)
+    bp_s = time.time() - t
+    t = time.time()
+    o.mlem(fp, np.ones(g.vol_shape, np.float32), 10, 1.0)
+    mlem_s = time.time() - t
+    return {"workload": "128x128x32, 256^2 detector, 15 views", "threads": nt,
+            "forward_s": round(fp_s, 3), "backprojection_s": round(bp_s, 3),
+            "mlem_10_iterations_s": round(mlem_s, 3),
+            "grays_per_s": round(g.nrays / (mlem_s / 10) / 1e9, 6)}
+
+
+def cpu_baseline(g, mu_h, data_h, seconds):
+    """Oracle FP+BP (mode R, OpenMP over rays / voxels, all host threads) on a
+    sample spread over all views and slices, extrapolated to one full
+    forward+back pass; the same on one thread with a quarter of the budget;
+    and the small workload in full."""
+    from oracle.oracle import Oracle
+
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.Generator(np.random.PCG64(7))
+    full_s, rays_done, n_rows, n_vox, spent = _cpu_sample(Oracle(g, nt), g, mu_h, data_h, seconds, rng)
+    one_s, _, rows1, vox1, spent1 = _cpu_sample(Oracle(g, 1), g, mu_h, data_h, seconds / 4, rng)
+    small = cpu_small_full(nt)
     return {
         "value": g.nrays / full_s / 1e9,
         "unit": "GRays/s",
         "cores": nt,
         "kind": "port",
+        "cpu_model": cpu_model(),
+        "online_cpus": os.cpu_count(),
         "sample": (f"oracle FP on {rays_done} rays ({n_rows} rows in each of {g.nviews} views) "
-                   f"+ BP on {n_vox} random voxels, {fp_s + bp_s:.1f} s, extrapolated to one "
+                   f"+ BP on {n_vox} random voxels, {spent:.1f} s, extrapolated to one "
                    f"full forward+back pass ({full_s:.0f} s)"),
         "seconds_per_step_extrapolated": full_s,
+        "one_core": {"value": g.nrays / one_s / 1e9, "unit": "GRays/s",
+                     "sample": f"{rows1} rows per view + {vox1} voxels on 1 thread, {spent1:.1f} s",
+                     "seconds_per_step_extrapolated": one_s},
+        "small_workload_full": small,
     }
 
 
@@ -373,8 +425,8 @@ def _load_cnn_params(name):
         return f["params"], float(f["scale"])
 
 
-def bench_learned_batched(args, torch, K, G):
-    """Learned plus batched at full size: `--recons` reconstructions of
+def learned_batched_runs(torch, K, G, n):
+    """Learned plus batched at full size: n reconstructions of
     1024x1024x64 / 25 views (phantom seeds 5000+i with 200
     microcalcifications, I0 = 5e3) in one mode-A batch (persistent launches,
     hipGraph replay).  Plain batched MLEM-A against the accelerated batched
@@ -389,7 +441,6 @@ def bench_learned_batched(args, torch, K, G):
     from kevernals_amd.trajectory import Schedule
 
     g = G.full_plain()
-    n = args.recons
     t = time.time()
     b = K.Batch([g] * n, mode=K.MODE_A)
     gen = torch.Generator(device="cuda")
@@ -427,11 +478,30 @@ def bench_learned_batched(args, torch, K, G):
     runs["plain"] = batched_gated_mlem(b, 100, params=None, references=refs)
     reset()
     runs["learned"] = batched_accelerated_mlem(b, sch, params=params, scale=scale, references=refs)
-    learned_rmse = [float(v) for v in runs["learned"].trace[-1][3]]
     reset()
     runs["schedule_only"] = batched_accelerated_mlem(b, sch, params=None, references=refs)
     out_t = _thresholds(runs, lambda e: (max(e[3]) if e[3] is not None else None), f"all {n}")
-    s1 = out_t["0.01"]
+    res = {
+        "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews,
+        "schedule": {"start": sch.start, "first": sch.first, "mid": sch.mid, "second": sch.second,
+                     "horizon": sch.horizon, "mlem_iterations": sch.cost},
+        "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
+        "time_to_rmse": out_t,
+        "gate": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
+        "reference": "100 batched MLEM-A iterations, FP32",
+        "reference_seconds": round(t_ref, 3),
+        "final_rmse": {"learned": [float(v) for v in runs["learned"].trace[-1][3]],
+                       "schedule_only": [float(v) for v in runs["schedule_only"].trace[-1][3]]},
+    }
+    del b, refs
+    return res, runs
+
+
+def bench_learned_batched(args, torch, K, G):
+    """The learned plus batched workload alone (learned_batched_runs) as the
+    JSON line's value: plain batched time / accelerated time to RMSE <= 1%."""
+    res, runs = learned_batched_runs(torch, K, G, args.recons)
+    s1 = res["time_to_rmse"]["0.01"]
     out = {
         "metric": METRIC,
         "value": s1["learned_speedup"],
@@ -442,16 +512,12 @@ def bench_learned_batched(args, torch, K, G):
         "dtype": "f32 (MLEM, jumps with FP64 Gram sums), bf16/f32-accumulate (CNN)",
         "data": "synthetic (seeded phantoms + 200 microcalcifications, Poisson I0=5e3)",
         "config": {"workload": "learned plus batched MLEM-A, time to RMSE<=1% vs 100-iteration MLEM",
-                   "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews,
-                   "schedule": {"start": sch.start, "first": sch.first, "mid": sch.mid, "second": sch.second,
-                                "horizon": sch.horizon, "mlem_iterations": sch.cost},
-                   "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
-                   "parallelism": "single GPU"},
-        "time_to_rmse": out_t,
-        "gate": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
-        "reference_seconds": round(t_ref, 3),
-        "final_rmse": {"learned": learned_rmse,
-                       "schedule_only": [float(v) for v in runs["schedule_only"].trace[-1][3]]},
+                   "reconstructions": res["reconstructions"], "volume": res["volume"], "views": res["views"],
+                   "schedule": res["schedule"], "cnn": res["cnn"], "parallelism": "single GPU"},
+        "time_to_rmse": res["time_to_rmse"],
+        "gate": res["gate"],
+        "reference_seconds": res["reference_seconds"],
+        "final_rmse": res["final_rmse"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -545,6 +611,66 @@ def bench_learned(args, torch, K, G):
     print(json.dumps(out), flush=True)
 
 
+def nnz_fixture(name):
+    """The oracle's accepted-pair counts (tests/golden/nnz/nnz.json)."""
+    path = os.path.join(ROOT, "tests", "golden", "nnz", "nnz.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def _event_ms(torch, fn, reps):
+    """Average device time of fn() over reps calls, HIP events on torch's
+    current stream (the context runs on it)."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_sart_saa(K, ctx, g, data, torch, saa_reps=5):
+    """The other two loops of the full-size workload: SART with relaxation
+    0.5 for 10 iterations (recons::ART, Reconstructors.h:19-134: fused
+    forward + residual, backprojection + update) and shift-and-add at the
+    volume's 64 plane heights (recons::ShiftAndAdd, :263-379; fulcrum at 0,
+    the projections being the workload's data)."""
+    x = torch.ones(g.vol_shape, device="cuda")
+    ctx.sart_iteration(x, data, 0.5)  # warm-up
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sart_ms = _event_ms(torch, lambda: ctx.sart_iteration(x, data, 0.5), 10)
+    sart_wall = (time.perf_counter() - t0) * 1e3 / 10
+    del x
+    planes = g.plane_heights()
+    sid = float(abs(g.det[0, 2] - np.float32(g.src[:, 2].mean(dtype=np.float32))))
+    out = torch.empty((len(planes), g.nv, g.nu), device="cuda")
+    ctx.shift_and_add(data, out, g.src[:, 1], planes, sid, 0.0, g.sy)
+    saa_ms = _event_ms(torch, lambda: ctx.shift_and_add(data, out, g.src[:, 1], planes, sid, 0.0, g.sy),
+                       saa_reps)
+    del out
+    plane_px = g.nu * g.nv * len(planes)
+    saa_bytes = 4 * g.nviews * plane_px + 4 * plane_px  # every view read per output pixel + the write
+    saa_gbs = saa_bytes / (saa_ms * 1e-3) / 1e9
+    log(f"[bench] SART {sart_ms:.3f} ms per iteration; shift-and-add ({len(planes)} planes) {saa_ms:.3f} ms")
+    return {
+        "sart": {"relaxation": 0.5, "iterations": 10, "ms_per_iteration": round(sart_ms, 4),
+                 "wall_ms_per_iteration": round(sart_wall, 4),
+                 "grays_per_s": round(g.nrays / (sart_ms * 1e-3) / 1e9, 4)},
+        "shift_and_add": {"planes": len(planes), "ms": round(saa_ms, 4),
+                          "roofline": {"bound": "hbm", "kernel": "saa_kernel",
+                                       "achieved": round(saa_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(saa_gbs / HBM_PEAK_GBS, 4),
+                                       "traffic": traffic_from_profiles("saa_kernel"),
+                                       "algorithmic_bytes": saa_bytes}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -552,6 +678,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="full workload: skip the learned-vs-plain time-to-RMSE measurement")
+    ap.add_argument("--no-loops", action="store_true", help="full workload: skip the SART and shift-and-add timing")
     ap.add_argument("--workload", choices=["full", "small", "batched", "learned", "learned-batched", "cnn", "io"],
                     default="full")
     ap.add_argument("--recons", type=int, default=8, help="reconstructions of the learned-batched workload")
@@ -583,8 +712,16 @@ def main():
     ctx = K.Context(0, g)
     ctx.use_torch_stream()
     mu, data = make_data(K, ctx, g, torch)
-    fp_pairs, bp_pairs = ctx.count_pairs(K.MODE_R)
+    kernel_pairs = ctx.count_pairs(K.MODE_R)
+    fp_pairs, bp_pairs = kernel_pairs
     log(f"[bench] accepted pairs: forward {fp_pairs}, backprojector {bp_pairs}")
+    oracle_pairs = nnz_fixture("full" if args.workload == "full" else "small")
+    if oracle_pairs is not None:
+        # the roofline numerators are the oracle's counts; the kernels' own
+        # counts are reported beside them (they are equal)
+        if (fp_pairs, bp_pairs) != (oracle_pairs["forward"], oracle_pairs["backprojector"]):
+            log("[bench] WARNING: kernel pair counts differ from the oracle's")
+        fp_pairs, bp_pairs = oracle_pairs["forward"], oracle_pairs["backprojector"]
 
     if args.copy_calibration:
         # reads 256 MiB and writes 256 MiB: the FETCH_SIZE / WRITE_SIZE of this
@@ -626,11 +763,31 @@ def main():
     achieved = kbytes / (kms * 1e-3) / 1e9
     value = nr / (ms_step * 1e-3) / 1e9
 
+    loops = None
+    if args.workload == "full" and not args.no_loops:
+        loops = bench_sart_saa(K, ctx, g, data, torch)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         cpu = cpu_baseline(g, mu.cpu().numpy(), data.cpu().numpy(), args.cpu_seconds)
         log(f"[bench] CPU baseline {cpu['value']:.3e} GRays/s on {cpu['cores']} threads")
+
+    learned = None
+    if args.workload == "full" and not args.no_learned:
+        del mu, data, x
+        ctx = None
+        torch.cuda.empty_cache()
+        res, _ = learned_batched_runs(torch, K, G, args.recons)
+        s1 = res["time_to_rmse"]["0.01"]
+        learned = {
+            "workload": (f"learned plus batched MLEM-A: {args.recons} reconstructions 1024x1024x64 / 25 views "
+                         "in one batch, time until every reconstruction is within 1% relative RMSE of its "
+                         "100-iteration MLEM-A result"),
+            "plain_s": s1["plain_s"], "learned_s": s1["learned_s"], "schedule_only_s": s1["schedule_only_s"],
+            "learned_speedup": s1["learned_speedup"], "schedule_only_speedup": s1["schedule_only_speedup"],
+            **res,
+        }
 
     out = {
         "metric": METRIC,
@@ -658,7 +815,9 @@ def main():
         },
         "mlem_iters_per_s": round(1e3 / ms_step, 3),
         "kernels_ms": {FP_KERNEL: round(fp_ms, 4), BP_KERNEL: round(bp_ms, 4)},
-        "accepted_pairs": {"forward": fp_pairs, "backprojector": bp_pairs},
+        "accepted_pairs": {"forward": fp_pairs, "backprojector": bp_pairs,
+                           "source": "oracle (tests/golden/nnz/nnz.json)" if oracle_pairs else "kernel",
+                           "kernel_count": {"forward": kernel_pairs[0], "backprojector": kernel_pairs[1]}},
         "roofline": {
             "bound": "hbm",
             "kernel": kname,
@@ -671,6 +830,10 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if loops is not None:
+        out.update(loops)
+    if learned is not None:
+        out["time_to_rmse_1pct"] = learned
     print(json.dumps(out), flush=True)
 
 

@@ -287,6 +287,52 @@ kv_status kv_log_scale(kv_context *ctx, const float *d_in, float *d_out, int64_t
 kv_status kv_traj_gram(kv_context *ctx, const float *const *d_iterates, int h, double *gram);
 kv_status kv_traj_combine(kv_context *ctx, const float *const *d_iterates, int h, const double *coef, float *d_out);
 
+/* Rayleigh-Ritz coefficients of a trajectory jump (host arithmetic, FP64):
+ * from the h x h Gram matrix of h log-ratios (kv_traj_gram) and npairs pairs
+ * (pairs[2k], pairs[2k+1]) = (i, j) with d_j = J d_i (consecutive log-ratios
+ * of one stretch of MLEM iterations; several stretches may contribute),
+ * coef[0..h-1] such that sum_i coef[i] d_i ~ p(J) J d_target with
+ * p(t) = sum_{k=1..horizon} t^k.  Gram directions of the pairs' first
+ * members below rel_tol times the largest eigenvalue are dropped; the Ritz
+ * values are clamped to [0, ritz_max] and returned in ritz (*nritz of them,
+ * at most npairs; either may be NULL).  Symmetric eigenproblems by cyclic
+ * Jacobi rotations.  No device work. */
+kv_status kv_ritz_coefficients(const double *gram, int h, const int32_t *pairs, int npairs, int target, int horizon,
+                               double rel_tol, double ritz_max, double *coef, double *ritz, int *nritz);
+
+/* ---- accelerated mode-A MLEM ----------------------------------------------
+ * No counterpart in the reference; the sibling of recons::MLEM
+ * (Reconstructors.h:136-253) in its mode-A form on line-integral data.
+ * Two-stretch jump schedule: MLEM-A runs `first` iterations, recording the
+ * iterates from `start` on (stretch A); a probe jumps to an estimate of
+ * iteration `mid` and runs `second` MLEM iterations from there (stretch B);
+ * the final jump starts again from iteration `first` with the Ritz basis of
+ * both stretches and lands on an estimate of iteration `horizon`.  The jump
+ * replaces the estimate only if its Poisson log-likelihood is not lower than
+ * iteration `first`'s; with CNN parameters (d_params, KV_CNN_NPARAMS floats
+ * on the device, input scale `scale`), the CNN step on the jump's result is
+ * kept only if it does not lower the jump's log-likelihood.  *kept (optional)
+ * receives the steps kept (0: iteration `first`, 1: the jump, 2: jump and CNN
+ * step).  Cost: first + second MLEM iterations, two Gram passes, two
+ * combinations, one likelihood walk.  Defaults: {2, 10, 40, 8, 100}.
+ * Allocates (first - start + second + 4) volumes of device memory for the
+ * call.  Synchronous. */
+typedef struct kv_accel_schedule {
+    int32_t start, first, mid, second, horizon;
+} kv_accel_schedule;
+/* d_vol: initial volume in (recons::MLEM's default is uniform 1), result out;
+ * d_data: line integrals (float[np][nv][nu]). */
+kv_status kv_mlem_accelerated(kv_context *ctx, float *d_vol, const float *d_data, const kv_accel_schedule *schedule,
+                              const float *d_params, float scale, int *kept);
+/* The same schedule on every reconstruction of a mode-A batch: the MLEM
+ * iterations of all reconstructions run as the batch's persistent launches;
+ * each reconstruction's jumps and gates run through its batch context.
+ * Volumes start from the batch's current contents; kept[i] (optional) per
+ * reconstruction.  Each result is bit-identical to kv_mlem_accelerated on
+ * that reconstruction alone.  Synchronous. */
+kv_status kv_batch_mlem_accelerated(kv_batch *b, const kv_accel_schedule *schedule, const float *d_params,
+                                    float scale, int32_t *kept);
+
 /* ---- host-buffer variants (synchronous; used by the C++ Projector) ------- */
 kv_status kv_forward_project_host(kv_context *ctx, const float *h_vol, float *h_proj, int mode);
 kv_status kv_back_project_host(kv_context *ctx, const float *h_proj, float *h_vol, int mode);

@@ -104,6 +104,10 @@ _SIGS = {
     "kv_log_scale": (_I, [_P, _P, _P, ctypes.c_int64, ctypes.c_double]),
     "kv_traj_gram": (_I, [_P, _P, _I, _P]),
     "kv_traj_combine": (_I, [_P, _P, _I, _P, _P]),
+    "kv_ritz_coefficients": (_I, [_P, _I, _P, _I, _I, _I, ctypes.c_double, ctypes.c_double, _P, _P,
+                                  ctypes.POINTER(_I)]),
+    "kv_mlem_accelerated": (_I, [_P, _P, _P, _P, _P, _F, ctypes.POINTER(_I)]),
+    "kv_batch_mlem_accelerated": (_I, [_P, _P, _P, _F, _P]),
 }
 
 
@@ -176,6 +180,34 @@ def cnn_workspace_offsets(shape):
     if st != KV_OK:
         raise KvError(st, lib.kv_last_error().decode())
     return list(o)
+
+
+class _KvAccelSchedule(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_int32), ("first", ctypes.c_int32), ("mid", ctypes.c_int32),
+                ("second", ctypes.c_int32), ("horizon", ctypes.c_int32)]
+
+
+def _schedule_struct(schedule) -> _KvAccelSchedule:
+    if schedule is None:
+        return _KvAccelSchedule(2, 10, 40, 8, 100)
+    return _KvAccelSchedule(schedule.start, schedule.first, schedule.mid, schedule.second, schedule.horizon)
+
+
+def ritz_coefficients_native(gram: np.ndarray, pairs, target: int, horizon: int, rel_tol: float,
+                             ritz_max: float):
+    """kv_ritz_coefficients: (coefficients over the h log-ratios, Ritz values)."""
+    lib = load_library()
+    G = np.ascontiguousarray(gram, dtype=np.float64)
+    h = G.shape[0]
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    coef = np.zeros(h)
+    ritz = np.zeros(len(pr))
+    nr = _I()
+    st = lib.kv_ritz_coefficients(_host_ptr(G), h, _host_ptr(pr), len(pr), int(target), int(horizon), rel_tol,
+                                  ritz_max, _host_ptr(coef), _host_ptr(ritz), ctypes.byref(nr))
+    if st != KV_OK:
+        raise KvError(st, lib.kv_last_error().decode())
+    return coef, ritz[:nr.value].copy()
 
 
 def _host_ptr(a: np.ndarray) -> int:
@@ -387,6 +419,17 @@ class Context:
             raise ValueError(f"{h} log-ratios need {h} coefficients, got {c.size}")
         self._dev(self.lib.kv_traj_combine, ctypes.cast(ptrs, ctypes.c_void_p), h, _host_ptr(c), _ptr(out))
 
+    def mlem_accelerated(self, vol, data, schedule=None, params=None, scale: float = 20.0) -> int:
+        """kv_mlem_accelerated: the two-stretch jump schedule (default
+        {2, 10, 40, 8, 100}) on vol (initial volume in, result out) with the
+        likelihood-gated CNN step when params (a device tensor of
+        CNN_NPARAMS floats) is given.  Returns the steps kept (0, 1, 2)."""
+        sch = _schedule_struct(schedule)
+        kept = _I()
+        self._dev(self.lib.kv_mlem_accelerated, _ptr(vol), _ptr(data), ctypes.byref(sch),
+                  _ptr(params) if params is not None else None, scale, ctypes.byref(kept))
+        return kept.value
+
     def forward_project_host(self, vol: np.ndarray, mode: int = MODE_R) -> np.ndarray:
         g = self.geometry
         vol = np.ascontiguousarray(vol, dtype=np.float32)
@@ -486,6 +529,16 @@ class Batch:
         ms = _F()
         self._check(self.lib.kv_batch_mlem(self.handle, _host_ptr(it), relax, int(bool(use_graph)), ctypes.byref(ms)))
         return ms.value
+
+    def mlem_accelerated(self, schedule=None, params=None, scale: float = 20.0):
+        """kv_batch_mlem_accelerated on every reconstruction (mode-A batch);
+        returns the steps kept per reconstruction."""
+        sch = _schedule_struct(schedule)
+        kept = np.zeros(len(self), dtype=np.int32)
+        self._check(self.lib.kv_batch_mlem_accelerated(self.handle, ctypes.byref(sch),
+                                                        _ptr(params) if params is not None else None, scale,
+                                                        _host_ptr(kept)))
+        return kept.tolist()
 
     def set_tiles(self, i: int, fp_layout: int, bp_slices: int):
         self._check(self.lib.kv_batch_set_tiles(self.handle, i, fp_layout, bp_slices))

@@ -444,4 +444,60 @@ Result<ImageDataPtr> ShiftAndAdd(TomoGeometry *p_tomoGeometry, ImageDataPtr p_pr
         return ReconstructorsErrorCode::ShiftAndAdd;
     return out;
 }
+Result<ImageDataPtr> MLEMAccelerated(TomoGeometry *p_tomoGeometry, ImageDataPtr p_lineIntegrals,
+                                     std::optional<ImageDataPtr> p_initialVolume,
+                                     const std::vector<float> *p_cnnParameters, float p_cnnScale,
+                                     AcceleratedSchedule p_schedule, int *p_stepsKept)
+{
+    const auto fail = ReconstructorsErrorCode::MLEM;
+    if (!p_tomoGeometry || p_lineIntegrals == nullptr)
+        return fail;
+    ImageDataPtr vol;
+    if (p_initialVolume.has_value()) {
+        vol = p_initialVolume.value();
+    } else {
+        PhantomMaker maker(p_tomoGeometry->GetVolume(), 1.f);
+        vol = maker.GetPhantom();
+    }
+    if (vol == nullptr || !vol->GetScalarPointer() || !p_lineIntegrals->GetScalarPointer())
+        return fail;
+    const Size3D vs = p_tomoGeometry->GetVolume()->GetSize3D();
+    const Size2D ds = p_tomoGeometry->projectionsRoisSize();
+    const int np = p_tomoGeometry->nbProjectionsRois();
+    if (!same_dims(vol, vs.x, vs.y, vs.z) || !same_dims(p_lineIntegrals, ds.x, ds.y, np))
+        return fail;
+    if (p_cnnParameters && p_cnnParameters->size() != (size_t)KV_CNN_NPARAMS)
+        return fail;
+    Projector projector{p_tomoGeometry};
+    kv_context *ctx = projector.context();
+    if (!ctx)
+        return fail;
+    const size_t nvox = (size_t)vs.x * vs.y * vs.z, nray = (size_t)ds.x * ds.y * np;
+    DevBuf dv, dd, dw;
+    if (!dv.alloc(ctx, nvox * 4) || !dd.alloc(ctx, nray * 4))
+        return fail;
+    if (kv_memcpy_h2d(ctx, dv.p, vol->GetScalarPointer(), nvox * 4) ||
+        kv_memcpy_h2d(ctx, dd.p, p_lineIntegrals->GetScalarPointer(), nray * 4))
+        return fail;
+    if (p_cnnParameters) {
+        if (!dw.alloc(ctx, (size_t)KV_CNN_NPARAMS * 4) ||
+            kv_memcpy_h2d(ctx, dw.p, p_cnnParameters->data(), (size_t)KV_CNN_NPARAMS * 4))
+            return fail;
+    }
+    const kv_accel_schedule sch{p_schedule.start, p_schedule.first, p_schedule.mid, p_schedule.second,
+                                p_schedule.horizon};
+    int kept = 0;
+    const kv_status st =
+        kv_mlem_accelerated(ctx, dv.f(), dd.f(), &sch, p_cnnParameters ? dw.f() : nullptr, p_cnnScale, &kept);
+    if (st != KV_OK) {
+        std::cout << "MLEMAccelerated: " << kv_last_error() << std::endl;
+        return fail;
+    }
+    if (kv_memcpy_d2h(ctx, vol->GetScalarPointer(), dv.p, nvox * 4))
+        return fail;
+    vol->Modified();
+    if (p_stepsKept)
+        *p_stepsKept = kept;
+    return vol;
+}
 }  // namespace recons

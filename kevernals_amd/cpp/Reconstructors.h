@@ -23,6 +23,7 @@
 #include <string>
 #include <system_error>
 #include <utility>
+#include <vector>
 
 #include "vtk_subset.h"
 
@@ -120,6 +121,27 @@ Result<ImageDataPtr> MLEM(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectio
 Result<ImageDataPtr> BackProjection(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectionImages);
 Result<ImageDataPtr> ShiftAndAdd(TomoGeometry *p_tomoGeometry, ImageDataPtr p_projectionImages,
                                  std::optional<std::string> p_outputDirectoryPath);
+
+// Accelerated MLEM, a sibling of MLEM with no counterpart in the reference:
+// MLEM on line-integral data in its matched (mode-A) form,
+// x <- x / A^T 1 * A^T(data / Ax), with the two-stretch trajectory-jump
+// schedule (kv_mlem_accelerated): `first` iterations recorded from `start`
+// on, a probe jump to iteration `mid` followed by `second` iterations, and a
+// final jump from iteration `first` to an estimate of iteration `horizon`,
+// kept only if it does not lower the Poisson log-likelihood.  With CNN
+// parameters (KV_CNN_NPARAMS floats, input scale p_cnnScale) the CNN step on
+// the jump's result is kept only if it does not lower the jump's
+// log-likelihood.  As MLEM: the initial volume defaults to uniform 1
+// (Reconstructors.h:152-153) and is otherwise updated in place and returned;
+// failures return ReconstructorsErrorCode::MLEM.  p_stepsKept (optional)
+// receives 0 (iteration `first` kept), 1 (the jump) or 2 (jump and CNN step).
+struct AcceleratedSchedule {
+    int start = 2, first = 10, mid = 40, second = 8, horizon = 100;
+};
+Result<ImageDataPtr> MLEMAccelerated(TomoGeometry *p_tomoGeometry, ImageDataPtr p_lineIntegrals,
+                                     std::optional<ImageDataPtr> p_initialVolume,
+                                     const std::vector<float> *p_cnnParameters = nullptr, float p_cnnScale = 20.f,
+                                     AcceleratedSchedule p_schedule = {}, int *p_stepsKept = nullptr);
 }  // namespace recons
 
 #ifndef KV_USE_REFERENCE_TREE

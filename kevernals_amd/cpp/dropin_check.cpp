@@ -8,9 +8,11 @@
 //   <dir>/geom.txt : nx ny nz sx sy sz nu nv np pu pv fulcrum_z, then per view
 //                    src_x src_y src_z det_x det_y det_z
 //   <dir>/vol.bin, <dir>/proj.bin : float32 volume and projections
+//   <dir>/ell.bin (optional): float32 line integrals for the accelerated
+//                    MLEM; <dir>/cnn.bin (optional): its CNN parameters
 // Writes fp.bin, bp.bin, mlem.bin, mlem_default.bin, art.bin, saa.bin,
 // mlem_verbose.bin and art_verbose.bin (with TIFF dumps under dump_mlem/ and
-// dump_art/) and
+// dump_art/), accel.bin and accel_cnn.bin (with ell.bin / cnn.bin) and
 // status.txt (one "name code" line per call).
 #include <cstdio>
 #include <filesystem>
@@ -21,6 +23,7 @@
 #include <vector>
 
 #include "Reconstructors.h"
+#include "kevernals_amd.h"
 
 namespace {
 bool read_raw(const std::string &path, float *dst, size_t n)
@@ -177,6 +180,29 @@ int main(int argc, char **argv)
         note("single_source", projector.lastError());
         if (!projector.lastError())
             write_raw(dir + "/single.bin", cont);
+    }
+
+    // accelerated MLEM on line integrals (<dir>/ell.bin), without and, given
+    // <dir>/cnn.bin (KV_CNN_NPARAMS floats), with the CNN step
+    {
+        auto ell = image(nu, nv, np, pu, pv, 1.0);
+        if (read_raw(dir + "/ell.bin", static_cast<float *>(ell->GetScalarPointer()), (size_t)nu * nv * np)) {
+            int kept = -1;
+            auto acc = recons::MLEMAccelerated(&geom, ell, std::nullopt, nullptr, 20.f, {}, &kept);
+            note("mlem_accelerated", acc.has_value() ? std::error_code() : acc.error());
+            if (acc)
+                write_raw(dir + "/accel.bin", acc.value());
+            status << "accel_kept " << kept << std::endl;
+            std::vector<float> params(KV_CNN_NPARAMS);
+            if (read_raw(dir + "/cnn.bin", params.data(), params.size())) {
+                kept = -1;
+                auto accc = recons::MLEMAccelerated(&geom, ell, std::nullopt, &params, 20.f, {}, &kept);
+                note("mlem_accelerated_cnn", accc.has_value() ? std::error_code() : accc.error());
+                if (accc)
+                    write_raw(dir + "/accel_cnn.bin", accc.value());
+                status << "accel_cnn_kept " << kept << std::endl;
+            }
+        }
     }
 
     // error path: one reconstructed slice (Reconstructors.h:274-279)

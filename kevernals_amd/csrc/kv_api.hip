@@ -574,8 +574,12 @@ kv_status kv_sensitivity(kv_context *ctx, float *d_vol)
     return KV_OK;
 }
 
-kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
-                           const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out)
+namespace {
+// Arguments and preconditions of recons::ShiftAndAdd (Reconstructors.h:
+// 274-305,319-323) and the row shifts t[k][p] = (int)(y_p (mZ - mF) / dy) of
+// Reconstructors.h:308-331, written to `shifts` when it is not null.
+kv_status saa_check(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes, const float *plane_z,
+                    float sid, float fulcrum_z, float voxel_dy, const float *d_out, int32_t *shifts)
 {
     if (kv_status st = check_ctx(ctx, true))
         return st;
@@ -584,7 +588,6 @@ kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *sr
         return KV_ERR_INVALID;
     }
     const KvGeom &g = ctx->g;
-    // Reconstructors.h:274-305
     if (nplanes <= 1) {
         kv_set_error("ShiftAndAdd: required nb of slices in reconstruction %d <= 1", nplanes);
         return KV_ERR_PRECONDITION;
@@ -601,9 +604,7 @@ kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *sr
         kv_set_error("ShiftAndAdd: voxel y spacing must be finite and positive");
         return KV_ERR_INVALID;
     }
-    // Reconstructors.h:308-331: per-plane integer row shifts
     const float mF = sid / (sid - fulcrum_z);
-    std::vector<int32_t> shifts((size_t)nplanes * g.np);
     for (int k = 0; k < nplanes; ++k) {
         const float z = plane_z[k];
         if (almost_equal_relative(z, sid)) {
@@ -619,15 +620,31 @@ kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *sr
                 kv_set_error("ShiftAndAdd: shift of plane %d view %d out of range", k, p);
                 return KV_ERR_INVALID;
             }
-            shifts[(size_t)k * g.np + p] = (int32_t)t;
+            if (shifts)
+                shifts[(size_t)k * g.np + p] = (int32_t)t;
         }
     }
+    return KV_OK;
+}
+}  // namespace
+
+kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
+                           const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out)
+{
+    if (kv_status st = check_ctx(ctx, true))
+        return st;
+    const KvGeom &g = ctx->g;
+    std::vector<int32_t> shifts((size_t)std::max(nplanes, 0) * g.np);
+    if (kv_status st = saa_check(ctx, d_proj, src_y, nplanes, plane_z, sid, fulcrum_z, voxel_dy, d_out,
+                                 shifts.data()))
+        return st;
+    const int kc = kv_saa_chunk(nplanes, g.np, [&](int k, int p) { return shifts[(size_t)k * g.np + p]; });
     int32_t *d_shifts = nullptr;
     if (kv_status st = dev_alloc((void **)&d_shifts, shifts.size() * 4))
         return st;
     hipError_t e = hipMemcpyAsync(d_shifts, shifts.data(), shifts.size() * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = kv_launch_saa(g.nu, g.nv, g.np, d_proj, d_shifts, nplanes, d_out, ctx->stream);
+        e = kv_launch_saa(g.nu, g.nv, g.np, d_proj, d_shifts, nplanes, kc, d_out, ctx->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_shifts);

@@ -223,6 +223,9 @@ void release(kv_batch *b)
 
 }  // namespace
 
+int kv_internal_batch_size(const kv_batch *b) { return b ? b->n : 0; }
+int kv_internal_batch_mode(const kv_batch *b) { return b ? b->mode : -1; }
+
 extern "C" {
 
 kv_status kv_batch_create(int device, int n, const kv_geometry *geoms, int mode, kv_batch **out)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -93,6 +94,10 @@ struct kv_context {
     float *d_ll = nullptr;     // projections of the further volumes of kv_log_likelihoods
 };
 
+// Reconstruction count and operator mode of a batch (0 / -1 for null).
+int kv_internal_batch_size(const kv_batch *b);
+int kv_internal_batch_mode(const kv_batch *b);
+
 // Thread-local error message.
 void kv_set_error(const char *fmt, ...);
 
@@ -177,7 +182,11 @@ hipError_t kv_launch_bp_batch(int mode, const KvProblem *probs, int nprob, long 
 hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 int kv_fp_batch_blocks_per_cu(int mode);
 int kv_bp_batch_blocks_per_cu(int mode);
-hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes,
+// Planes per chunk of the shift-and-add kernel for these shifts (shift(k, p)
+// = row shift of plane k, view p): the largest of 8, 4, 2 whose staged rows
+// fit the kernel's LDS, else 1.
+int kv_saa_chunk(int nplanes, int np, const std::function<int(int, int)> &shift);
+hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes, int kc,
                          float *out, hipStream_t s);
 hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblocks, hipStream_t s);
 hipError_t kv_launch_loglik(const KvGeom &g, const float *data, const float *ax, double *partials, int nblocks,

@@ -1,6 +1,9 @@
 // Shift-and-add and the deterministic log-likelihood reduction.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+#include <functional>
+
 #include "kv_internal.h"
 
 namespace {
@@ -8,26 +11,86 @@ namespace {
 // recons::ShiftAndAdd (Reconstructors.h:314-378): for plane k,
 // out(u, v) = sum_p (1/P) * proj_p(u, v + t[k][p]), zero outside the
 // detector (vtkImageConstantPad), summed in view order in FP64 and stored as
-// FP32.  One lane per (u, v); the plane loop keeps the rows a lane reads
-// close together (t changes slowly with k).
+// FP32.
+//
+// A workgroup owns a strip of SAA_TU detector columns by SAA_RV rows and
+// walks the planes in chunks of kc (<= SAA_KC).  For each chunk it visits the
+// views in order; the rows of view p that the chunk's planes read,
+// [v0 + min_k t, v0 + SAA_RV - 1 + max_k t], are staged once in LDS as their
+// FP64 products x * (1/P) (0 outside the detector), and every (plane, row)
+// of the chunk adds its staged product: each projection row is read from
+// memory once per chunk instead of once per plane, and the product is formed
+// once per staged element.  The sums keep the view order and the
+// per-element arithmetic of the one-lane-per-pixel form, so the result is
+// bit-identical.  The host picks kc so that SAA_RV plus the chunk's largest
+// shift spread fits the SAA_SR staged rows (kc = 1 always fits).
+constexpr int SAA_TU = 64, SAA_RV = 16, SAA_KC = 8, SAA_SR = 64;
+constexpr int SAA_RPT = SAA_RV / 4;  // output rows per thread (4 wavefronts)
+
 __global__ __launch_bounds__(256) void saa_kernel(int nu, int nv, int np, const float *__restrict__ proj,
-                                                  const int32_t *__restrict__ shifts, int nplanes,
+                                                  const int32_t *__restrict__ shifts, int nplanes, int kc,
                                                   float *__restrict__ out)
 {
-    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (u >= nu || v >= nv)
-        return;
+    __shared__ double st[SAA_SR][SAA_TU];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int u = blockIdx.x * SAA_TU + lane;
+    const int v0 = blockIdx.y * SAA_RV;
     const double wgt = 1.0 / (double)np;
     const size_t plane = (size_t)nu * nv;
-    for (int k = 0; k < nplanes; ++k) {
-        double acc = 0.0;
+    for (int k0 = 0; k0 < nplanes; k0 += kc) {
+        const int nk = min(kc, nplanes - k0);
+        double acc[SAA_KC][SAA_RPT];
+#pragma unroll
+        for (int j = 0; j < SAA_KC; ++j)
+#pragma unroll
+            for (int i = 0; i < SAA_RPT; ++i)
+                acc[j][i] = 0.0;
         for (int p = 0; p < np; ++p) {
-            const int vv = v + shifts[k * np + p];
-            const double x = (vv >= 0 && vv < nv) ? (double)proj[p * plane + (size_t)vv * nu + u] : 0.0;
-            acc += x * wgt;
+            // the chunk's shifts for view p (wave-uniform)
+            int t[SAA_KC];
+            int tlo = shifts[k0 * np + p], thi = tlo;
+#pragma unroll
+            for (int j = 0; j < SAA_KC; ++j) {
+                t[j] = j < nk ? shifts[(k0 + j) * np + p] : t[0];
+                tlo = min(tlo, t[j]);
+                thi = max(thi, t[j]);
+            }
+            // <= SAA_SR by the host's choice of kc; the clamps below keep
+            // every LDS access inside the array whatever the shifts
+            const int nrows = min(SAA_RV + thi - tlo, SAA_SR);
+            __syncthreads();  // the previous view's readers are done
+            const float *src = proj + (size_t)p * plane;
+            for (int r = grp; r < nrows; r += 4) {
+                const int vv = v0 + tlo + r;
+                double x = 0.0;
+                if (vv >= 0 && vv < nv && u < nu)
+                    x = (double)src[(size_t)vv * nu + u] * wgt;
+                st[r][lane] = x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < SAA_KC; ++j) {
+                if (j < nk) {
+                    const int r0 = min(grp * SAA_RPT + t[j] - tlo, SAA_SR - SAA_RPT);
+#pragma unroll
+                    for (int i = 0; i < SAA_RPT; ++i)
+                        acc[j][i] += st[r0 + i][lane];
+                }
+            }
         }
-        out[k * plane + (size_t)v * nu + u] = (float)acc;
+        if (u < nu) {
+#pragma unroll
+            for (int j = 0; j < SAA_KC; ++j) {
+                if (j < nk) {
+#pragma unroll
+                    for (int i = 0; i < SAA_RPT; ++i) {
+                        const int v = v0 + grp * SAA_RPT + i;
+                        if (v < nv)
+                            out[(size_t)(k0 + j) * plane + (size_t)v * nu + u] = (float)acc[j][i];
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -226,11 +289,33 @@ hipError_t kv_launch_sum(int64_t n, const float *x, double *partials, int nblock
     return hipGetLastError();
 }
 
-hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes, float *out,
-                         hipStream_t s)
+int kv_saa_chunk(int nplanes, int np, const std::function<int(int, int)> &shift)
 {
-    const dim3 grid((nu + 63) / 64, (nv + 3) / 4), block(256);
-    saa_kernel<<<grid, block, 0, s>>>(nu, nv, np, proj, shifts, nplanes, out);
+    for (int kc = SAA_KC; kc > 1; kc /= 2) {
+        int spread = 0;
+        for (int k0 = 0; k0 < nplanes; k0 += kc)
+            for (int p = 0; p < np; ++p) {
+                int lo = shift(k0, p), hi = lo;
+                for (int k = k0 + 1; k < std::min(nplanes, k0 + kc); ++k) {
+                    const int t = shift(k, p);
+                    lo = std::min(lo, t);
+                    hi = std::max(hi, t);
+                }
+                spread = std::max(spread, hi - lo);
+            }
+        if (SAA_RV + spread <= SAA_SR)
+            return kc;
+    }
+    return 1;
+}
+
+hipError_t kv_launch_saa(int nu, int nv, int np, const float *proj, const int32_t *shifts, int nplanes, int kc,
+                         float *out, hipStream_t s)
+{
+    if (kc < 1 || kc > SAA_KC)
+        return hipErrorInvalidValue;
+    const dim3 grid((nu + SAA_TU - 1) / SAA_TU, (nv + SAA_RV - 1) / SAA_RV), block(256);
+    saa_kernel<<<grid, block, 0, s>>>(nu, nv, np, proj, shifts, nplanes, kc, out);
     return hipGetLastError();
 }
 

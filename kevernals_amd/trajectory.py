@@ -13,8 +13,8 @@ returns) give the Rayleigh-Ritz projection T of J, and the remaining path
     p(t) = sum_{i=1..N} t^i,
 is approximated by B Q p(T) T (coordinates of d_{h-2}) in the W-orthonormal
 basis B Q.  The volume passes (Gram matrix, combination) are the HIP kernels
-behind kv_traj_gram / kv_traj_combine; the (h-1) x (h-1) eigenproblems here
-are host arithmetic in FP64.
+behind kv_traj_gram / kv_traj_combine; the (h-1) x (h-1) eigenproblems are
+the library's host arithmetic in FP64 (kv_ritz_coefficients).
 
 This has no counterpart in the reference; it accelerates recons::MLEM in its
 mode-A form (Reconstructors.h:182-242).
@@ -26,7 +26,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import TRAJ_MAX, Context
+from . import TRAJ_MAX, Context, ritz_coefficients_native
 
 
 def ritz_coefficients(gram: np.ndarray, pairs, target: int, horizon: int, rel_tol: float = 1e-11,
@@ -43,7 +43,8 @@ def ritz_coefficients(gram: np.ndarray, pairs, target: int, horizon: int, rel_to
     B^T W B whose eigenvalues exceed rel_tol times the largest (the rest is
     rounding noise of FP32 iterates), T = Q^T (B^T W JB) Q symmetrised, Ritz
     values clamped to [0, ritz_max] (EM's Jacobian has its spectrum in
-    [0, 1))."""
+    [0, 1)).  The eigenproblems are solved by the library's host code
+    (kv_ritz_coefficients, cyclic Jacobi in FP64)."""
     G = np.asarray(gram, dtype=np.float64)
     h = G.shape[0]
     if G.shape != (h, h) or h < 2:
@@ -53,28 +54,13 @@ def ritz_coefficients(gram: np.ndarray, pairs, target: int, horizon: int, rel_to
     pairs = [(int(i), int(j)) for i, j in pairs]
     if not pairs or any(not (0 <= i < h and 0 <= j < h) for i, j in pairs):
         raise ValueError("pairs must index the Gram matrix")
-    bi = [i for i, _ in pairs]
-    bj = [j for _, j in pairs]
-    if target not in bi:
+    if target not in [i for i, _ in pairs]:
         raise ValueError("the target must be the first member of a pair")
-    Gb = G[np.ix_(bi, bi)]
-    H = G[np.ix_(bi, bj)]
-    ev, U = np.linalg.eigh(Gb)
-    coef = np.zeros(h)
-    if not ev.size or ev[-1] <= 0.0 or horizon == 0:
-        return coef, np.zeros(0)
-    keep = ev > ev[-1] * rel_tol
-    Q = U[:, keep] / np.sqrt(ev[keep])
-    T = Q.T @ H @ Q
-    T = 0.5 * (T + T.T)
-    th, Y = np.linalg.eigh(T)
-    th = np.clip(th, 0.0, ritz_max)
-    p = th * (1.0 - th ** horizon) / (1.0 - th)
-    c = Q.T @ Gb[:, bi.index(target)]
-    cb = Q @ (Y @ (p * th * (Y.T @ c)))
-    for k, i in enumerate(bi):
-        coef[i] += cb[k]
-    return coef, th
+    if h > TRAJ_MAX:
+        raise ValueError(f"at most {TRAJ_MAX} log-ratios")
+    # the host solve of the library (kv_ritz_coefficients), which the C++
+    # accelerated reconstruction uses too
+    return ritz_coefficients_native(G, pairs, target, horizon, rel_tol, ritz_max)
 
 
 def extrapolation_coefficients(gram: np.ndarray, horizon: int, rel_tol: float = 1e-11,

@@ -50,3 +50,33 @@ def combine(iterates, coef):
         e += c * di
     y = np.where(xh.ravel() > 0, xh.ravel() * np.exp(e), xh.ravel())
     return y.reshape(xh.shape)
+
+
+def ritz_coefficients(gram, pairs, target, horizon, rel_tol=1e-11, ritz_max=1.0 - 1e-5):
+    """The Rayleigh-Ritz coefficient solve restated with numpy's symmetric
+    eigensolver (LAPACK), the checker of the library's own solve
+    (kv_ritz_coefficients, cyclic Jacobi): coefficients c over the h
+    log-ratios with sum_i c_i d_i ~ p(J) J d_target, p(t) = sum_{k=1..N} t^k,
+    and the Ritz values."""
+    G = np.asarray(gram, dtype=np.float64)
+    h = G.shape[0]
+    bi = [int(i) for i, _ in pairs]
+    bj = [int(j) for _, j in pairs]
+    Gb = G[np.ix_(bi, bi)]
+    H = G[np.ix_(bi, bj)]
+    ev, U = np.linalg.eigh(Gb)
+    coef = np.zeros(h)
+    if not ev.size or ev[-1] <= 0.0 or horizon == 0:
+        return coef, np.zeros(0)
+    keep = ev > ev[-1] * rel_tol
+    Q = U[:, keep] / np.sqrt(ev[keep])
+    T = Q.T @ H @ Q
+    T = 0.5 * (T + T.T)
+    th, Y = np.linalg.eigh(T)
+    th = np.clip(th, 0.0, ritz_max)
+    p = th * (1.0 - th ** horizon) / (1.0 - th)
+    c = Q.T @ Gb[:, bi.index(target)]
+    cb = Q @ (Y @ (p * th * (Y.T @ c)))
+    for k, i in enumerate(bi):
+        coef[i] += cb[k]
+    return coef, th

@@ -167,3 +167,42 @@ def test_runs_that_stop_early_return_the_estimate(K):
     for _ in range(2):
         ctx.mlem_iteration(y, data, 1.0, K.MODE_A)
     np.testing.assert_array_equal(more.x.cpu().numpy(), y.cpu().numpy())
+
+
+def test_c_abi_accelerated_mlem_is_bitwise_the_python_schedule(K):
+    """kv_mlem_accelerated (the C++ sequence over the C ABI, Ritz solve in
+    the library) against learned.accelerated_mlem, with and without the
+    committed network, and kv_batch_mlem_accelerated against both: every
+    result bit-identical, the same steps kept."""
+    import os
+
+    from kevernals_amd import geometry as G
+    from kevernals_amd.learned import Cnn, accelerated_mlem
+
+    wpath = os.path.join(os.path.dirname(K.__file__), "weights", "cnn_schedule_to_100.npz")
+    with np.load(wpath, allow_pickle=False) as f:
+        params, scale = f["params"], float(f["scale"])
+    d_params = torch.from_numpy(params.copy()).cuda()
+    specs = [((70, 37, 11), (0.6, 0.6, 2.0), (150, 90), (0.5, 0.5), 7, 15.0),
+             ((64, 64, 24), (0.5, 0.5, 1.5), (128, 128), (0.5, 0.5), 5, 12.0)]
+    gs = [G.arc_geometry(*s) for s in specs]
+    probs = [problem(K, g, 8500 + i) for i, g in enumerate(gs)]
+    for use_cnn in (False, True):
+        print(f"[accel] C ABI vs Python schedule, cnn={use_cnn}", flush=True)
+        singles = []
+        for i, (ctx, data) in enumerate(probs):
+            py = accelerated_mlem(ctx, data, cnn=Cnn(ctx, params, scale) if use_cnn else None)
+            x = torch.ones(gs[i].vol_shape, device="cuda")
+            kept = ctx.mlem_accelerated(x, data, params=d_params if use_cnn else None, scale=scale)
+            np.testing.assert_array_equal(x.cpu().numpy(), py.x.cpu().numpy(), err_msg=f"problem {i}")
+            assert kept == py.accepted
+            singles.append((x.cpu().numpy(), kept))
+        b = K.Batch(gs, mode=K.MODE_A)
+        for i, (ctx, data) in enumerate(probs):
+            b.set_volume(i, np.ones(gs[i].vol_shape, np.float32))
+            b.set_data(i, data.cpu().numpy())
+        kept_b = b.mlem_accelerated(params=d_params if use_cnn else None, scale=scale)
+        for i in range(len(gs)):
+            np.testing.assert_array_equal(b.get_volume(i), singles[i][0], err_msg=f"batched problem {i}")
+            assert kept_b[i] == singles[i][1]
+        b.close()

@@ -148,3 +148,44 @@ def test_dropin_from_xml_matches_oracle(tmp_path, oracle_mod):
     rc, saa = oracle_mod.saa(proj, gm.src[:, 1], planes, float(sid), fz, gm.sy)
     assert rc == 0
     np.testing.assert_array_equal(rd("saa.bin", (len(planes), gm.nv, gm.nu)), saa)
+
+
+def test_dropin_accelerated_mlem_matches_python_schedule(tmp_path):
+    """recons::MLEMAccelerated (C++ over kv_mlem_accelerated) against
+    learned.accelerated_mlem on the marshalled geometry: the same volume bit
+    for bit and the same steps kept, without and with the committed CNN."""
+    import torch
+
+    import kevernals_amd as K
+    from kevernals_amd import geometry as G
+    from kevernals_amd.learned import Cnn, accelerated_mlem, transmission_data
+    from kevernals_amd.phantom import phantom
+
+    assert os.path.exists(CHECK), "drop-in not built"
+    g = G.arc_geometry((96, 80, 16), (0.5, 0.5, 2.0), (192, 160), (0.5, 0.5), 9, 15.0)
+    gm, sz, dz = marshalled(g)
+    ctx = K.Context(0, gm)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(8600)
+    data = transmission_data(ctx, torch.from_numpy(phantom(gm, 8600)).cuda(), 1e4, gen)
+    wpath = os.path.join(os.path.dirname(K.__file__), "weights", "cnn_schedule_to_100.npz")
+    with np.load(wpath, allow_pickle=False) as f:
+        params, scale = f["params"].astype(np.float32), float(f["scale"])
+    assert scale == 20.0  # the drop-in check passes the default scale
+    with open(tmp_path / "geom.txt", "w") as f:
+        f.write(f"{g.nx} {g.ny} {g.nz} {g.sx!r} {g.sy!r} {g.sz!r} {g.nu} {g.nv} {g.nviews} {g.pu!r} {g.pv!r} 0.0\n")
+        for p in range(g.nviews):
+            f.write(" ".join(f"{float(v):.9g}" for v in [*g.src[p], *g.det[p]]) + "\n")
+    phantom(gm, 5).tofile(tmp_path / "vol.bin")
+    np.zeros(gm.proj_shape, np.float32).tofile(tmp_path / "proj.bin")
+    data.cpu().numpy().tofile(tmp_path / "ell.bin")
+    params.tofile(tmp_path / "cnn.bin")
+    r = subprocess.run([CHECK, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    status = dict(line.split() for line in open(tmp_path / "status.txt"))
+    assert status["mlem_accelerated"] == "0" and status["mlem_accelerated_cnn"] == "0"
+    for name, cnn in (("accel", None), ("accel_cnn", Cnn(ctx, params, scale))):
+        run = accelerated_mlem(ctx, data, cnn=cnn)
+        got = np.fromfile(tmp_path / f"{name}.bin", dtype=np.float32).reshape(gm.vol_shape)
+        np.testing.assert_array_equal(got, run.x.cpu().numpy(), err_msg=name)
+        assert int(status[f"{name}_kept"]) == run.accepted

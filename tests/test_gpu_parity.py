@@ -281,6 +281,34 @@ def test_mlem_a_against_fp64_oracle_and_loglik(K, O):
     assert all(b >= a - 1e-7 * abs(a) for a, b in zip(lls, lls[1:]))
 
 
+def test_mlem_a_100_iterations_fp32_reference_vs_fp64(K, O):
+    """The time-to-RMSE references are 100 MLEM-A iterations in FP32 on the
+    GPU; the target names a 100-iteration FP64 MLEM.  Here both run on the
+    same seeded data (64x64x16, 128^2 detector, 15 views, Poisson I0 1e4):
+    the FP32 result's relative RMSE against the FP64 one bounds what the
+    substitution moves a 1 % threshold by."""
+    g = half_small()
+    o = O.Oracle(g)
+    from kevernals_amd.phantom import poisson_line_integrals
+
+    data = poisson_line_integrals(o.fp_a(phantom_of(g, 41)), 1e4, 42)
+    ctx = K.Context(0, g)
+    d = dev(data)
+    x = torch.ones(g.vol_shape, device="cuda")
+    for _ in range(100):
+        ctx.mlem_iteration(x, d, 1.0, K.MODE_A)
+    got = host(x).astype(np.float64)
+    print("[parity] oracle: 100 FP64 MLEM-A iterations ...", flush=True)
+    ref = o.mlem_a64(data.astype(np.float64), np.ones(g.vol_shape), 100)
+    rmse = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    x10 = o.mlem_a64(data.astype(np.float64), np.ones(g.vol_shape), 10)
+    span = float(np.linalg.norm(x10 - ref) / np.linalg.norm(ref))
+    print(f"[parity] 100 MLEM-A iterations, FP32 GPU vs FP64 oracle: relative RMSE {rmse:.2e} "
+          f"(iteration 10 vs 100: {span:.2e})", flush=True)
+    assert rmse < 1e-3  # a tenth of the 1 % threshold
+    assert span > 10 * rmse
+
+
 def test_log_likelihoods_share_one_walk_bit_for_bit(K, O):
     """log_likelihoods of 1..3 volumes (one ray walk) equals log_likelihood
     of each volume bit for bit, and each matches the FP64 oracle."""

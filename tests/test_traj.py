@@ -122,3 +122,25 @@ def test_two_stretch_jump_exact_on_linear_recurrence(lams):
         np.testing.assert_allclose(np.log(y), np.log(evolve(L, lam, x0, 10 + horizon)), rtol=0, atol=2e-6)
     with pytest.raises(ValueError):  # two iterates in stretch B: no J-pair there
         two_stretch_coefficients(traj.gram(its[:11], sens), 9, 10)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_native_solve_matches_lapack(seed):
+    """The library's Jacobi solve (kv_ritz_coefficients, behind
+    trajectory.ritz_coefficients) against numpy's LAPACK eigensolver in the
+    oracle, on Gram matrices of noisy recurrences with one and two stretches:
+    the Ritz values and the coefficients agree to rounding."""
+    from kevernals_amd.trajectory import ritz_coefficients
+
+    r = rng(100 + seed)
+    x, sens = recurrence(3000, [0.4, 0.7, 0.9, 0.97, 0.995], 17, seed)
+    its = [x(k) * (1.0 + 1e-7 * r.standard_normal(3000)) for k in range(18)]
+    G = traj.gram(its, sens)
+    for pairs, target, horizon in (([(i, i + 1) for i in range(16)], 15, 60),
+                                   ([(i, i + 1) for i in range(7)] + [(i, i + 1) for i in range(9, 16)], 6, 90)):
+        got, ritz = ritz_coefficients(G, pairs, target, horizon)
+        want, ritz_w = traj.ritz_coefficients(G, pairs, target, horizon)
+        assert ritz.shape == ritz_w.shape
+        np.testing.assert_allclose(ritz, ritz_w, rtol=0, atol=1e-9)
+        scale = np.abs(want).max()
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-7 * scale)
