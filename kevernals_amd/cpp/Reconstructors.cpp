@@ -10,7 +10,10 @@
 #include "kevernals_amd.h"
 
 #ifdef KV_USE_REFERENCE_TREE
+#ifdef KV_USE_REAL_VTK
+// the reference's own PhantomMaker (its header needs VTK)
 #include "modules/dataHandling/PhantomMaker.h"
+#endif
 #else
 // ------------------------------------------------------------ error codes --
 namespace {
@@ -231,6 +234,35 @@ vtkSmartPointer<vtkImageData> Projector::PerformBackProjection(vtkSmartPointer<v
     }
     return out;
 }
+
+#if defined(KV_USE_REFERENCE_TREE) && !defined(KV_USE_REAL_VTK)
+namespace {
+// Inside the reference tree without VTK the reference's PhantomMaker header
+// cannot be included; this is its GetPhantom with no shapes
+// (PhantomMaker.cpp:9-51): every voxel at the background density.
+class PhantomMaker {
+public:
+    PhantomMaker(TomoVolume *p_volume, float p_backgroundDensity) : m_volume(p_volume), m_background(p_backgroundDensity)
+    {
+    }
+    ImageDataPtr GetPhantom() const
+    {
+        if (!m_volume)
+            return nullptr;
+        const Size3D s = m_volume->GetSize3D();
+        const VoxelSpacing sp = m_volume->GetVoxelSpacing();
+        auto img = new_image(s.x, s.y, s.z, sp.x, sp.y, sp.z);
+        float *d = static_cast<float *>(img->GetScalarPointer());
+        std::fill(d, d + points(img), m_background);
+        return img;
+    }
+
+private:
+    TomoVolume *m_volume;
+    float m_background;
+};
+}  // namespace
+#endif
 
 #ifndef KV_USE_REFERENCE_TREE
 ImageDataPtr PhantomMaker::GetPhantom() const
