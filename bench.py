@@ -38,7 +38,7 @@ def log(msg):
 def make_data(K, ctx, g, torch, seed=2):
     """Seeded phantom with 200 microcalcifications; Poisson transmission data
     at I0 = 5e3; reference-discretisation data p = FP_R(mu) * l_hat / l."""
-    from kevernals_amd.phantom import phantom, poisson_line_integrals
+    from kevernals_amd.phantom import phantom
 
     t = time.time()
     mu_h = phantom(g, seed, microcalcifications=200)
@@ -46,7 +46,8 @@ def make_data(K, ctx, g, torch, seed=2):
     ell = torch.empty(g.proj_shape, device="cuda")
     ctx.forward_project(mu, ell, K.MODE_A)
     torch.cuda.synchronize()
-    ell_hat = torch.from_numpy(poisson_line_integrals(ell.cpu().numpy(), 5e3, seed + 1000)).cuda()
+    ell_hat = torch.empty_like(ell)
+    ctx.poisson_data(ell, ell_hat, 5e3, seed + 1000)
     fp = torch.empty(g.proj_shape, device="cuda")
     ctx.forward_project(mu, fp, K.MODE_R)
     data = torch.where(ell > 0, fp * ell_hat / torch.clamp(ell, min=1e-30), torch.zeros_like(fp))
@@ -443,12 +444,10 @@ def learned_batched_runs(torch, K, G, n):
     g = G.full_plain()
     t = time.time()
     b = K.Batch([g] * n, mode=K.MODE_A)
-    gen = torch.Generator(device="cuda")
     for i in range(n):
         ctx = b.context(i)
-        gen.manual_seed(5000 + i)
         mu = torch.from_numpy(phantom(g, 5000 + i, microcalcifications=200)).cuda()
-        data = transmission_data(ctx, mu, 5e3, gen)
+        data = transmission_data(ctx, mu, 5e3, 5000 + i)
         ctx.copy(b.buffers(i)[1], data, data.numel() * 4)
         torch.cuda.synchronize()
         del mu, data
@@ -540,10 +539,8 @@ def bench_learned(args, torch, K, G):
 
     g = G.full_plain()
     ctx = K.Context(0, g)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(5000)
     mu = torch.from_numpy(phantom(g, 5000, microcalcifications=200)).cuda()
-    data = transmission_data(ctx, mu, 5e3, gen)
+    data = transmission_data(ctx, mu, 5e3, 5000)
     del mu
     ref = torch.ones(g.vol_shape, device="cuda")
     torch.cuda.synchronize()

@@ -269,6 +269,17 @@ kv_status kv_phantom(kv_context *ctx, float background, const kv_pave *paves, in
  * and stored as FP32.  n values; d_out may be d_in. */
 kv_status kv_log_scale(kv_context *ctx, const float *d_in, float *d_out, int64_t n, double c);
 
+/* Seeded transmission data (no counterpart in the reference, which reads
+ * measured projections): for each of the n rays, counts
+ * I ~ Poisson(i0 * exp(-d_ell[i])) and d_out[i] = -ln(clamp(I, 1, i0) / i0)
+ * (FP64, stored as FP32); d_ell holds mode-A line integrals in mm.  Draws
+ * come from Philox4x64-10 (counter (i, block, 0, 0), key (seed, 0x6b76),
+ * four uniforms (w >> 11) * 2^-53 per block); means below 10 are sampled by
+ * inversion, larger ones by Hoermann's PTRS.  exp, log and log k! are
+ * evaluated with IEEE +, -, *, / only, so results do not depend on a math
+ * library.  d_out may be d_ell.  Asynchronous on the context's stream. */
+kv_status kv_poisson_data(kv_context *ctx, const float *d_ell, int64_t n, float i0, uint64_t seed, float *d_out);
+
 /* ---- trajectory extrapolation of mode-A MLEM ------------------------------
  * No counterpart in the reference: these serve the accelerated
  * reconstruction built on the mode-A MLEM update (a jump from iteration k to

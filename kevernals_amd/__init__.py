@@ -103,6 +103,7 @@ _SIGS = {
     "kv_phantom": (_I, [_P, _F, _P, _I, _P, _I, _P]),
     "kv_log_scale": (_I, [_P, _P, _P, ctypes.c_int64, ctypes.c_double]),
     "kv_traj_gram": (_I, [_P, _P, _I, _P]),
+    "kv_poisson_data": (_I, [_P, _P, ctypes.c_int64, _F, ctypes.c_uint64, _P]),
     "kv_traj_combine": (_I, [_P, _P, _I, _P, _P]),
     "kv_ritz_coefficients": (_I, [_P, _I, _P, _I, _I, _I, ctypes.c_double, ctypes.c_double, _P, _P,
                                   ctypes.POINTER(_I)]),
@@ -394,6 +395,12 @@ class Context:
         sp = np.ascontiguousarray(np.asarray(spheres, dtype=np.float32).reshape(-1, 5))
         self._dev(self.lib.kv_phantom, float(background), _host_ptr(pv) if len(pv) else None, len(pv),
                   _host_ptr(sp) if len(sp) else None, len(sp), _ptr(vol))
+
+    def poisson_data(self, ell, out, i0: float, seed: int):
+        """out = -ln(clamp(I, 1, i0)/i0), I ~ Poisson(i0 exp(-ell)) per ray,
+        from the library's seeded Philox4x64-10 / PTRS sampler."""
+        self._dev(self.lib.kv_poisson_data, _ptr(ell), int(ell.numel()), float(i0), int(seed) & (2**64 - 1),
+                  _ptr(out))
 
     def log_scale(self, src, dst, c: float = 1.0):
         """dst = c*log(1 + src) (sign-symmetric), the reference's data

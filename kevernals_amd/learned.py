@@ -102,17 +102,19 @@ def mlem_a(ctx: Context, x, data, iters: int):
         ctx.mlem_iteration(x, data, 1.0, MODE_A)
 
 
-def transmission_data(ctx: Context, mu, i0: float, generator):
+def transmission_data(ctx: Context, mu, i0: float, seed: int):
     """Noisy line integrals -ln(clamp(I, 1, I0)/I0), I ~ Poisson(I0 exp(-A mu)),
-    sampled on the device.  Counts above I0 are clamped: the line integrals
-    stay non-negative, as the multiplicative MLEM update requires (negative
-    data drive estimates negative and the iteration unstable)."""
+    sampled on the device by the library's seeded generator
+    (kv_poisson_data: Philox4x64-10 streams, inversion / PTRS).  Counts
+    above I0 are clamped: the line integrals stay non-negative, as the
+    multiplicative MLEM update requires (negative data drive estimates
+    negative and the iteration unstable)."""
     torch = __import__("torch")
     g = ctx.geometry
     ell = torch.empty(g.proj_shape, device="cuda")
     ctx.forward_project(mu, ell, MODE_A)
-    counts = torch.poisson(i0 * torch.exp(-ell), generator=generator)
-    return -torch.log(torch.clamp(counts, min=1.0, max=i0) / i0)
+    ctx.poisson_data(ell, ell, i0, seed)
+    return ell
 
 
 def rmse(x, ref) -> float:

@@ -5,8 +5,9 @@ source/modules/dataHandling/PhantomMaker.cpp:9-51): a background density
 plus the density of every shape that contains the voxel's lower corner
 (TomoVolume.cpp:219-230 grid positions), shapes adding up.
 
-Transmission data: I ~ Poisson(I0 * exp(-l)), l the mode-A line integral in
-mm, and the measured line integral l_hat = -ln(max(I, 1) / I0).
+Transmission data (I ~ Poisson(I0 * exp(-l)), l the mode-A line integral in
+mm, l_hat = -ln(clamp(I, 1, I0) / I0)) is sampled on the device by
+kv_poisson_data (Context.poisson_data, learned.transmission_data).
 """
 from __future__ import annotations
 
@@ -90,13 +91,3 @@ def phantom(g: Geometry, seed: int, microcalcifications: int = 0) -> np.ndarray:
     """Seeded phantom: 0.02/mm background, 20 ellipsoids adding U[0.01,0.05]/mm,
     plus optional 0.1/mm microcalcification spheres of 0.3-1 mm."""
     return rasterise(g, shapes(g, seed, n_spheres=microcalcifications))
-
-
-def poisson_line_integrals(ell: np.ndarray, i0: float, seed: int) -> np.ndarray:
-    """Noisy line integrals -ln(clamp(I, 1, I0)/I0), I ~ Poisson(I0 exp(-ell)).
-    Counts above I0 (short rays) are clamped so that the line integrals stay
-    non-negative, as the multiplicative MLEM update requires."""
-    rng = _rng(seed)
-    lam = i0 * np.exp(-ell.astype(np.float64))
-    counts = rng.poisson(lam)
-    return (-np.log(np.clip(counts, 1, i0) / i0)).astype(np.float32)

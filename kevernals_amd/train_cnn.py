@@ -41,7 +41,7 @@ def log(msg):
     print(msg, flush=True)
 
 
-def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref, gen):
+def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref):
     """(input, target) pairs: the trajectory extrapolation made at iteration
     `at` from the last `history` log-ratios, jumping to iteration k_ref,
     paired with the k_ref-iteration estimate of the same run."""
@@ -56,7 +56,7 @@ def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref, gen):
     t0 = time.time()
     for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
         mu = torch.from_numpy(mu_h).cuda()
-        data = transmission_data(ctx, mu, i0, gen)
+        data = transmission_data(ctx, mu, i0, seeds[n])
         x = torch.ones(g.vol_shape, device="cuda")
         tr.reset()
         if at - history <= 0:
@@ -76,7 +76,7 @@ def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref, gen):
     return xs, ts
 
 
-def make_schedule_pairs(ctx, seeds, pool, i0, k_ref, gen):
+def make_schedule_pairs(ctx, seeds, pool, i0, k_ref):
     """(input, target) pairs: the final estimate of the jump schedule (jumps
     not gated) and the k_ref-iteration estimate of the same data."""
     import torch
@@ -90,7 +90,7 @@ def make_schedule_pairs(ctx, seeds, pool, i0, k_ref, gen):
     t0 = time.time()
     for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
         mu = torch.from_numpy(mu_h).cuda()
-        data = transmission_data(ctx, mu, i0, gen)
+        data = transmission_data(ctx, mu, i0, seeds[n])
         jm = Jumper(ctx, g.vol_shape, sch)
         x = torch.ones(g.vol_shape, device="cuda")
         if sch.start == 0:
@@ -116,7 +116,7 @@ def make_schedule_pairs(ctx, seeds, pool, i0, k_ref, gen):
     return xs, ts
 
 
-def make_pairs(ctx, seeds, pool, i0, stages, k_ref, gen):
+def make_pairs(ctx, seeds, pool, i0, stages, k_ref):
     """(input, target) pairs: the MLEM-A estimate after each iteration count
     in `stages`, paired with the estimate after k_ref iterations of the same
     run (the target tensor is shared by a phantom's pairs)."""
@@ -129,7 +129,7 @@ def make_pairs(ctx, seeds, pool, i0, stages, k_ref, gen):
     t0 = time.time()
     for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
         mu = torch.from_numpy(mu_h).cuda()
-        data = transmission_data(ctx, mu, i0, gen)
+        data = transmission_data(ctx, mu, i0, seeds[n])
         x = torch.ones(g.vol_shape, device="cuda")
         snaps = []
         done = 0
@@ -179,24 +179,22 @@ def main(argv=None):
 
     g = G.learned_train()
     ctx = K.Context(0, g)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(args.seed + 12345)
     t0 = time.time()
     stages = sorted({int(v) for v in args.stages.split(",")}) if args.stages else [args.k_in]
     assert 0 < stages[0] and stages[-1] < args.k_ref
     train_seeds = list(range(TRAIN_SEED0, TRAIN_SEED0 + args.pairs))
     val_seeds = list(range(VAL_SEED0, VAL_SEED0 + args.val))
     if args.input == "krylov":
-        xs, ts = make_krylov_pairs(ctx, train_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
-        vx, vt = make_krylov_pairs(ctx, val_seeds, pool, args.i0, args.at, args.history, args.k_ref, gen)
+        xs, ts = make_krylov_pairs(ctx, train_seeds, pool, args.i0, args.at, args.history, args.k_ref)
+        vx, vt = make_krylov_pairs(ctx, val_seeds, pool, args.i0, args.at, args.history, args.k_ref)
         stages = [args.at]
     elif args.input == "schedule":
-        xs, ts = make_schedule_pairs(ctx, train_seeds, pool, args.i0, args.k_ref, gen)
-        vx, vt = make_schedule_pairs(ctx, val_seeds, pool, args.i0, args.k_ref, gen)
+        xs, ts = make_schedule_pairs(ctx, train_seeds, pool, args.i0, args.k_ref)
+        vx, vt = make_schedule_pairs(ctx, val_seeds, pool, args.i0, args.k_ref)
         stages = [0]
     else:
-        xs, ts = make_pairs(ctx, train_seeds, pool, args.i0, stages, args.k_ref, gen)
-        vx, vt = make_pairs(ctx, val_seeds, pool, args.i0, [args.k_in], args.k_ref, gen)
+        xs, ts = make_pairs(ctx, train_seeds, pool, args.i0, stages, args.k_ref)
+        vx, vt = make_pairs(ctx, val_seeds, pool, args.i0, [args.k_in], args.k_ref)
     pool.close()
     pool.join()
     t_data = time.time() - t0

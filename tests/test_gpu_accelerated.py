@@ -29,10 +29,8 @@ def problem(K, g, seed):
     from kevernals_amd.phantom import phantom
 
     ctx = K.Context(0, g)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
     mu = torch.from_numpy(phantom(g, seed)).cuda()
-    data = transmission_data(ctx, mu, 1e4, gen)
+    data = transmission_data(ctx, mu, 1e4, seed)
     torch.cuda.synchronize()
     return ctx, data
 

@@ -51,14 +51,15 @@ def say(msg):
 
 def test_small_workload_mlem_10_iterations_bit_exact(K, oracle_mod):
     from kevernals_amd import geometry as G
-    from kevernals_amd.phantom import phantom, poisson_line_integrals
+    from kevernals_amd.phantom import phantom
+    from oracle.rng import poisson_data
 
     g = G.small_plain()
     o = oracle_mod.Oracle(g, oracle_mod.threads())
     mu = phantom(g, 1)
     # reference-discretisation data from Poisson line integrals (I0 = 1e4)
     ell = o.fp_a(mu)
-    ell_hat = poisson_line_integrals(ell, 1e4, 1001)
+    ell_hat = poisson_data(ell, 1e4, 1001)
     fp = o.fp_r(mu)
     data = np.where(ell > 0, fp * ell_hat / np.maximum(ell, np.float32(1e-30)), 0).astype(np.float32)
     ctx = K.Context(0, g)
@@ -204,8 +205,7 @@ def test_learned_inference_configuration(K):
 
     g = G.learned_infer()
     ctx = K.Context(0, g)
-    gen = torch.Generator(device="cuda").manual_seed(6000)
-    data = transmission_data(ctx, dev(phantom(g, 6000)), 1e4, gen)
+    data = transmission_data(ctx, dev(phantom(g, 6000)), 1e4, 6000)
     x = torch.ones(g.vol_shape, device="cuda")
     for k in range(1, 101):
         ctx.mlem_iteration(x, data, 1.0, K.MODE_A)
@@ -242,7 +242,6 @@ def test_learned_batched_full_size(K):
     g = G.full_plain()
     n = 8
     b = K.Batch([g] * n, mode=K.MODE_A)
-    gen = torch.Generator(device="cuda")
     blf = np.array(g.blf())
     fov = np.array([g.sx * g.nx, g.sy * g.ny, g.sz * g.nz])
     datas = []
@@ -254,8 +253,7 @@ def test_learned_batched_full_size(K):
         ctx = b.context(i)
         mu = torch.empty(g.vol_shape, device="cuda")
         ctx.phantom(mu, 0.02, paves, spheres)
-        gen.manual_seed(5000 + i)
-        data = transmission_data(ctx, mu, 5e3, gen)
+        data = transmission_data(ctx, mu, 5e3, 5000 + i)
         ctx.copy(b.buffers(i)[1], data, data.numel() * 4)
         torch.cuda.synchronize()
         if i == 3:

@@ -38,10 +38,8 @@ def problem(K, seed, geometry):
     from kevernals_amd.phantom import phantom
 
     ctx = K.Context(0, geometry)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
     mu = torch.from_numpy(phantom(geometry, seed)).cuda()
-    data = transmission_data(ctx, mu, 1e4, gen)
+    data = transmission_data(ctx, mu, 1e4, seed)
     return ctx, data
 
 

@@ -260,10 +260,10 @@ def test_mode_a_sequences_match_oracle(K, O):
 def test_mlem_a_against_fp64_oracle_and_loglik(K, O):
     g = half_small()
     o = O.Oracle(g)
-    from kevernals_amd.phantom import poisson_line_integrals
+    from oracle.rng import poisson_data
 
     mu = phantom_of(g, 4)
-    data = poisson_line_integrals(o.fp_a(mu), 1e4, 5)
+    data = poisson_data(o.fp_a(mu), 1e4, 5)
     ctx = K.Context(0, g)
     x = dev(np.ones(g.vol_shape, np.float32))
     d = dev(data)
@@ -289,9 +289,9 @@ def test_mlem_a_100_iterations_fp32_reference_vs_fp64(K, O):
     substitution moves a 1 % threshold by."""
     g = half_small()
     o = O.Oracle(g)
-    from kevernals_amd.phantom import poisson_line_integrals
+    from oracle.rng import poisson_data
 
-    data = poisson_line_integrals(o.fp_a(phantom_of(g, 41)), 1e4, 42)
+    data = poisson_data(o.fp_a(phantom_of(g, 41)), 1e4, 42)
     ctx = K.Context(0, g)
     d = dev(data)
     x = torch.ones(g.vol_shape, device="cuda")
@@ -314,9 +314,9 @@ def test_log_likelihoods_share_one_walk_bit_for_bit(K, O):
     of each volume bit for bit, and each matches the FP64 oracle."""
     g = half_small()
     o = O.Oracle(g)
-    from kevernals_amd.phantom import poisson_line_integrals
+    from oracle.rng import poisson_data
 
-    data = poisson_line_integrals(o.fp_a(phantom_of(g, 4)), 1e4, 5)
+    data = poisson_data(o.fp_a(phantom_of(g, 4)), 1e4, 5)
     ctx = K.Context(0, g)
     d = dev(data)
     vols = [dev(phantom_of(g, s) + np.float32(0.001)) for s in (5, 6, 7)]

@@ -345,10 +345,11 @@ def test_mlem_fixed_point_where_all_rays_are_consistent(oracle_mod):
 def test_mlem_a_increases_loglik_and_keeps_positivity(oracle_mod):
     g = G.scaled_down(G.small_plain(), 4)
     o = oracle_mod.Oracle(g, 8)
-    from kevernals_amd.phantom import phantom, poisson_line_integrals
+    from kevernals_amd.phantom import phantom
+    from oracle.rng import poisson_data
 
     mu = phantom(g, 11)
-    data = poisson_line_integrals(o.fp_a(mu), 1e4, 12).astype(np.float64)
+    data = poisson_data(o.fp_a(mu), 1e4, 12).astype(np.float64)
     x = np.ones(g.vol_shape)
     ll = [o.loglik_a64(data, x)]
     for _ in range(4):
