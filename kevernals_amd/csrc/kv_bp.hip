@@ -178,7 +178,8 @@ constexpr int XBW = 32;          // x band width (detector columns per plane)
 constexpr int XBS = XBW + 1;     // its LDS row stride (odd: conflict-free)
 constexpr int YBR = 96;          // y band rows
 
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+// lo < hi and v not NaN: the median of the three is the clamp
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 
 // min/max as signed-integer min/max of the IEEE bit patterns.  They order two
 // floats exactly as fminf/fmaxf when both are >= +0, and put every negative
@@ -446,7 +447,10 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             // accumulates (Projector.cu:727-728)
             float a = accs[zi][0][tid], wsm = accs[zi][1][tid];
             const int ncol = hxw - lx + 1;
-            const bool in_band = lx - xb0 >= 0 && hxw - xb0 < XBW && lx - xb1 >= 0 && hxw - xb1 < XBW;
+            // lx - xb >= 0 and hxw - xb < XBW for both planes' bands, as two
+            // unsigned compares (meaningful for the ncol <= 4 it is used with)
+            const bool in_band = (unsigned)(lx - xb0) <= (unsigned)(XBW - ncol) &&
+                                 (unsigned)(lx - xb1) <= (unsigned)(XBW - ncol);
             const bool rows_in_band = ly - ybase >= 0 && hyw - ybase < YBR;  // wave-uniform
             if (ncol <= 4 && in_band && rows_in_band) {
                 // x intervals of the (at most NC) columns stay in registers;
@@ -475,29 +479,27 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                     // weight is negative) and reads past the plane return 0
                     // instead of faulting.  The next row's values are
                     // requested before the current row is reduced.
-                    float v[NC], vn[NC];
+                    // two register sets alternate between rows (no copies)
+                    float va[NC], vb[NC];
 #pragma unroll
                     for (int c = 0; c < NC; ++c)
-                        v[c] = vn[c] = 0.f;
+                        va[c] = vb[c] = 0.f;
                     int off = (ly * g.nu + lx) * 4;
-                    if (EP != KV_EP_COUNT && !UNIT) {
+                    constexpr bool LOADS = EP != KV_EP_COUNT && !UNIT;
+                    auto load_row = [&](float (&dst)[NC]) {
+                        if (LOADS) {
 #pragma unroll
-                        for (int c = 0; c < NC; ++c)
-                            v[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
-                    }
-                    for (int py = ly; py <= hyw; ++py) {
+                            for (int c = 0; c < NC; ++c)
+                                dst[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
+                        }
+                    };
+                    auto row = [&](int py, const float (&v)[NC]) {
                         const int r = py - ybase;
                         const float a1y = yr0[r], a2y = yr1[r];
                         // max/min are exact: the reference's grouping
                         // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
                         const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
                         const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
-                        off += 4 * g.nu;
-                        if (EP != KV_EP_COUNT && !UNIT) {
-#pragma unroll
-                            for (int c = 0; c < NC; ++c)
-                                vn[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
-                        }
 #pragma unroll
                         for (int c = 0; c < NC; ++c) {
                             const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
@@ -518,10 +520,21 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                                 a = ok ? a + w * v[c] : a;
                             }
                         }
-#pragma unroll
-                        for (int c = 0; c < NC; ++c)
-                            v[c] = vn[c];
+                    };
+                    // each row's values are requested before the previous
+                    // row is reduced
+                    load_row(va);
+                    int py = ly;
+                    for (; py < hyw; py += 2) {
+                        off += 4 * g.nu;
+                        load_row(vb);
+                        row(py, va);
+                        off += 4 * g.nu;
+                        load_row(va);
+                        row(py + 1, vb);
                     }
+                    if (py == hyw)
+                        row(py, va);
                 };
                 if (__all(ncol <= 2))
                     rows(std::integral_constant<int, 2>{});

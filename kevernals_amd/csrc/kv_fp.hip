@@ -175,6 +175,27 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     if (cx == 0) ax = kInf;
     if (cy == 0) ay = kInf;
     if (cz == 0) az = kInf;
+    // Steps of the main loops, where m is the minimum head (FAST): only the
+    // first run in x, y, z order whose head equals m advances.  A tie
+    // between runs is then consumed over several steps, each extra step a
+    // zero-width segment that Projector.cu:358-359 rejects, so the accepted
+    // segments are unchanged and one run's increment and end threshold are
+    // selected instead of stepping all three.
+    auto advance1 = [&](float m) {
+        const bool px = ax == m;
+        const bool py = !px && ay == m;
+        const bool pz = !px && !py;
+        // selects of values (a conditional of the captured lvalues would
+        // select their addresses and load through them)
+        const float vix = ix, viy = iy, viz = iz, vtx = tx, vty = ty, vtz = tz;
+        const float inc = px ? vix : (py ? viy : viz);
+        const float thr = px ? vtx : (py ? vty : vtz);
+        float nv = m + inc;
+        nv = nv > thr ? kInf : nv;
+        ax = px ? nv : ax;
+        ay = py ? nv : ay;
+        az = pz ? nv : az;
+    };
     // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
     auto advance = [&](float m) {
         const bool px = ax == m, py = ay == m, pz = az == m;
@@ -225,15 +246,34 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         filler_left = false;
     }
     advance(prev);
-    // entries below the fillers' value (all of them when there is no filler)
+    // entries below the fillers' value (all of them when there is no filler);
+    // two entries per turn, so that the previous entry needs no copy
     const float lim = filler_left ? fill : kInf;
-    for (;;) {
-        const float cur = min3f(ax, ay, az);
-        if (!(cur < lim))
-            break;
-        advance(cur);
-        segment(prev, cur);
-        prev = cur;
+    if (FAST) {
+        for (;;) {
+            const float c1 = min3f(ax, ay, az);
+            if (!(c1 < lim))
+                break;
+            advance1(c1);
+            segment(prev, c1);
+            const float c2 = min3f(ax, ay, az);
+            if (!(c2 < lim)) {
+                prev = c1;
+                break;
+            }
+            advance1(c2);
+            segment(c1, c2);
+            prev = c2;
+        }
+    } else {
+        for (;;) {
+            const float cur = min3f(ax, ay, az);
+            if (!(cur < lim))
+                break;
+            advance(cur);
+            segment(prev, cur);
+            prev = cur;
+        }
     }
     if (filler_left) {
         // the 1.0 entry (run values equal to it are the same entry), then

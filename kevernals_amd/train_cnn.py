@@ -30,11 +30,20 @@ TRAIN_SEED0 = 4000
 VAL_SEED0 = 6000
 
 
-def _phantom(args):
+GEOMETRY = "train"  # set by main before the workers fork
+
+
+def _geometry():
     from kevernals_amd import geometry as G
+
+    return G.full_plain() if GEOMETRY == "full" else G.learned_train()
+
+
+def _phantom(args):
     from kevernals_amd.phantom import phantom
 
-    return phantom(G.learned_train(), args)
+    # the full-size workload's phantoms carry 200 microcalcifications
+    return phantom(_geometry(), args, microcalcifications=200 if GEOMETRY == "full" else 0)
 
 
 def log(msg):
@@ -76,16 +85,17 @@ def make_krylov_pairs(ctx, seeds, pool, i0, at, history, k_ref):
     return xs, ts
 
 
-def make_schedule_pairs(ctx, seeds, pool, i0, k_ref):
+def make_schedule_pairs(ctx, seeds, pool, i0, k_ref, second=8):
     """(input, target) pairs: the final estimate of the jump schedule (jumps
-    not gated) and the k_ref-iteration estimate of the same data."""
+    not gated; `second` probe iterations) and the k_ref-iteration estimate of
+    the same data."""
     import torch
 
     from kevernals_amd.learned import transmission_data
     from kevernals_amd.trajectory import Jumper, Schedule
 
     g = ctx.geometry
-    sch = Schedule(horizon=k_ref)
+    sch = Schedule(horizon=k_ref, second=second)
     xs, ts = [], []
     t0 = time.time()
     for n, mu_h in enumerate(pool.imap(_phantom, seeds, chunksize=4)):
@@ -164,10 +174,20 @@ def main(argv=None):
     ap.add_argument("--at", type=int, default=20, help="--input krylov: iteration of the jump")
     ap.add_argument("--history", type=int, default=18, help="--input krylov: log-ratios used")
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--geometry", choices=["train", "full"], default="train",
+                    help="full: the full-size workload (1024x1024x64, 25 views, 200 microcalcifications)")
+    ap.add_argument("--second", type=int, default=8, help="--input schedule: probe iterations")
+    ap.add_argument("--crop", default=None,
+                    help="train on random crops nz,ny,nx of the pairs (e.g. 32,256,256)")
+    ap.add_argument("--crops-per-pair", type=int, default=4)
+    ap.add_argument("--lr-decay-epoch", type=int, default=0,
+                    help="from this epoch on (1-based, 0: never) the learning rate is divided by 3")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="cnn_weights.npz")
     args = ap.parse_args(argv)
 
+    global GEOMETRY
+    GEOMETRY = args.geometry
     # CPU workers first: no process is started after the GPU is initialised
     pool = mp.get_context("fork").Pool(args.workers)
 
@@ -177,7 +197,7 @@ def main(argv=None):
     from kevernals_amd import geometry as G
     from kevernals_amd.learned import Cnn
 
-    g = G.learned_train()
+    g = _geometry()
     ctx = K.Context(0, g)
     t0 = time.time()
     stages = sorted({int(v) for v in args.stages.split(",")}) if args.stages else [args.k_in]
@@ -189,8 +209,8 @@ def main(argv=None):
         vx, vt = make_krylov_pairs(ctx, val_seeds, pool, args.i0, args.at, args.history, args.k_ref)
         stages = [args.at]
     elif args.input == "schedule":
-        xs, ts = make_schedule_pairs(ctx, train_seeds, pool, args.i0, args.k_ref)
-        vx, vt = make_schedule_pairs(ctx, val_seeds, pool, args.i0, args.k_ref)
+        xs, ts = make_schedule_pairs(ctx, train_seeds, pool, args.i0, args.k_ref, args.second)
+        vx, vt = make_schedule_pairs(ctx, val_seeds, pool, args.i0, args.k_ref, args.second)
         stages = [0]
     else:
         xs, ts = make_pairs(ctx, train_seeds, pool, args.i0, stages, args.k_ref)
@@ -214,11 +234,19 @@ def main(argv=None):
     rng = np.random.Generator(np.random.PCG64(args.seed))
     history = []
     t1 = time.time()
+    crop = [int(v) for v in args.crop.split(",")] if args.crop else None
     for ep in range(args.epochs):
         order = rng.permutation(len(xs))
         tl = []
+        lr = args.lr / 3 if args.lr_decay_epoch and ep + 1 >= args.lr_decay_epoch else args.lr
         for i in order:
-            tl.append(cnn.train_step(xs[i], ts[i], args.lr))
+            if crop is None:
+                tl.append(cnn.train_step(xs[i], ts[i], lr))
+                continue
+            for _ in range(args.crops_per_pair):
+                z0, y0, x0 = (int(rng.integers(0, n - c + 1)) for n, c in zip(g.vol_shape, crop))
+                sl = (slice(z0, z0 + crop[0]), slice(y0, y0 + crop[1]), slice(x0, x0 + crop[2]))
+                tl.append(cnn.train_step(xs[i][sl].contiguous(), ts[i][sl].contiguous(), lr))
         vl = val_loss()
         history.append((ep + 1, float(np.mean(tl)), vl))
         log(f"[train] epoch {ep + 1}: train {np.mean(tl):.5e} val {vl:.5e} ({time.time() - t1:.1f} s)")
@@ -226,7 +254,8 @@ def main(argv=None):
     cnn.save(args.out, history=np.array(history), identity_val=base, pairs=len(xs), phantoms=args.pairs,
              k_in=args.k_in, stages=np.array(stages), k_ref=args.k_ref, i0=args.i0, data_seconds=t_data,
              input_kind=args.input, at=args.at, krylov_history=args.history,
-             steps_per_s=steps_per_s)
+             steps_per_s=steps_per_s, geometry=args.geometry, second=args.second,
+             crop=np.array(crop if crop else [0, 0, 0]), crops_per_pair=args.crops_per_pair)
     log(f"[train] saved {args.out}: {cnn.step} steps, {steps_per_s:.1f} steps/s")
     return 0
 
