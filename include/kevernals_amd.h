@@ -143,6 +143,24 @@ kv_status kv_apply_update(kv_context *ctx, int loop, float *d_vol, const float *
 kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
                            const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out);
 
+/* kv_shift_and_add without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The source y positions, plane heights and the row-shift table are staged in
+ * a fixed array that the library holds on each GPU until the process ends
+ * ((KV_SAA_NOALLOC_MAX_VIEWS + KV_SAA_NOALLOC_MAX_PLANES +
+ * KV_SAA_NOALLOC_MAX_SHIFTS) * 4 bytes = 276 KiB); a call with more than
+ * KV_SAA_NOALLOC_MAX_VIEWS views, KV_SAA_NOALLOC_MAX_PLANES planes or
+ * KV_SAA_NOALLOC_MAX_SHIFTS plane-view pairs returns KV_ERR_INVALID.  The
+ * array is shared by every call on a GPU and keeps the last call's contents:
+ * calls that overlap on one GPU (from several threads or streams) race.
+ * Output bit-identical to kv_shift_and_add; synchronous, like it. */
+#define KV_SAA_NOALLOC_MAX_VIEWS 1024
+#define KV_SAA_NOALLOC_MAX_PLANES 4096
+#define KV_SAA_NOALLOC_MAX_SHIFTS 65536
+kv_status kv_shift_and_add_noalloc(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
+                                   const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out);
+
 /* Poisson log-likelihood sum over rays with Ax > 0 of data*log(Ax) - Ax,
  * Ax the mode-A forward projection of d_vol; reduced deterministically in
  * FP64.  Synchronous (writes *out on return). */

@@ -76,6 +76,7 @@ _SIGS = {
     "kv_count_pairs": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "kv_sart_iteration": (_I, [_P, _P, _P, _F]),
     "kv_shift_and_add": (_I, [_P, _P, _P, _I, _P, _F, _F, _F, _P]),
+    "kv_shift_and_add_noalloc": (_I, [_P, _P, _P, _I, _P, _F, _F, _F, _P]),
     "kv_log_likelihood": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
     "kv_log_likelihoods": (_I, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "kv_sensitivity": (_I, [_P, _P]),
@@ -337,11 +338,14 @@ class Context:
     def sensitivity(self, vol):
         self._dev(self.lib.kv_sensitivity, _ptr(vol))
 
-    def shift_and_add(self, proj, out, src_y, plane_z, sid: float, fulcrum_z: float, voxel_dy: float):
+    def shift_and_add(self, proj, out, src_y, plane_z, sid: float, fulcrum_z: float, voxel_dy: float,
+                      noalloc: bool = False):
+        """kv_shift_and_add, or with noalloc=True kv_shift_and_add_noalloc
+        (no device allocation; staging limits KV_SAA_NOALLOC_MAX_*)."""
         sy = np.ascontiguousarray(src_y, dtype=np.float32)
         pz = np.ascontiguousarray(plane_z, dtype=np.float32)
-        self._dev(self.lib.kv_shift_and_add, _ptr(proj), _host_ptr(sy), len(pz), _host_ptr(pz), sid, fulcrum_z,
-                  voxel_dy, _ptr(out))
+        fn = self.lib.kv_shift_and_add_noalloc if noalloc else self.lib.kv_shift_and_add
+        self._dev(fn, _ptr(proj), _host_ptr(sy), len(pz), _host_ptr(pz), sid, fulcrum_z, voxel_dy, _ptr(out))
 
     def log_likelihood(self, data, vol) -> float:
         out = ctypes.c_double()

@@ -222,6 +222,7 @@ kv_status kv_create(int device, kv_context **out)
         return KV_ERR_INVALID;
     }
     KV_TRY_HIP(hipSetDevice(device));
+    KV_TRY_HIP(kv_internal_scratch_init(device));
     auto *ctx = new kv_context();
     ctx->device = device;
     const hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
@@ -577,7 +578,8 @@ kv_status kv_sensitivity(kv_context *ctx, float *d_vol)
 namespace {
 // Arguments and preconditions of recons::ShiftAndAdd (Reconstructors.h:
 // 274-305,319-323) and the row shifts t[k][p] = (int)(y_p (mZ - mF) / dy) of
-// Reconstructors.h:308-331, written to `shifts` when it is not null.
+// Reconstructors.h:308-331, written to `shifts` when it is not null (the
+// no-allocation variant evaluates them on the device instead).
 kv_status saa_check(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes, const float *plane_z,
                     float sid, float fulcrum_z, float voxel_dy, const float *d_out, int32_t *shifts)
 {
@@ -648,6 +650,53 @@ kv_status kv_shift_and_add(kv_context *ctx, const float *d_proj, const float *sr
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_shifts);
+    if (e != hipSuccess) {
+        kv_set_error("shift-and-add failed: %s", hipGetErrorString(e));
+        return KV_ERR_DEVICE;
+    }
+    return KV_OK;
+}
+
+kv_status kv_shift_and_add_noalloc(kv_context *ctx, const float *d_proj, const float *src_y, int nplanes,
+                                   const float *plane_z, float sid, float fulcrum_z, float voxel_dy, float *d_out)
+{
+    if (kv_status st = saa_check(ctx, d_proj, src_y, nplanes, plane_z, sid, fulcrum_z, voxel_dy, d_out, nullptr))
+        return st;
+    const KvGeom &g = ctx->g;
+    if (g.np > KV_SAA_NOALLOC_MAX_VIEWS || nplanes > KV_SAA_NOALLOC_MAX_PLANES ||
+        (int64_t)g.np * nplanes > KV_SAA_NOALLOC_MAX_SHIFTS) {
+        kv_set_error("shift-and-add without allocation: %d views x %d planes exceeds the staging limits "
+                     "(%d views, %d planes, %d view-plane pairs)",
+                     g.np, nplanes, KV_SAA_NOALLOC_MAX_VIEWS, KV_SAA_NOALLOC_MAX_PLANES, KV_SAA_NOALLOC_MAX_SHIFTS);
+        return KV_ERR_INVALID;
+    }
+    int32_t *scr = kv_internal_scratch(ctx->device);
+    if (!scr) {
+        kv_set_error("shift-and-add without allocation: staging array of device %d not resolved", ctx->device);
+        return KV_ERR_STATE;
+    }
+    float *d_sy = (float *)scr;
+    float *d_pz = (float *)(scr + KV_SAA_NOALLOC_MAX_VIEWS);
+    int32_t *d_shifts = scr + KV_SAA_NOALLOC_MAX_VIEWS + KV_SAA_NOALLOC_MAX_PLANES;
+    const float mF = sid / (sid - fulcrum_z);
+    // the shifts the device evaluates, recomputed here (same expressions) to
+    // pick the kernel's plane chunk
+    const int kc = kv_saa_chunk(nplanes, g.np, [&](int k, int p) {
+        const float mZ = sid / (sid - plane_z[k]);
+        const float coeff = mZ - mF;
+        const float world = src_y[p] * coeff;
+        return (int)(int32_t)(world / voxel_dy);
+    });
+    hipError_t e = hipMemcpyAsync(d_sy, src_y, (size_t)g.np * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_pz, plane_z, (size_t)nplanes * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = kv_launch_saa_shifts(g.np, nplanes, sid, mF, voxel_dy, d_sy, d_pz, d_shifts, ctx->stream);
+    if (e == hipSuccess)
+        e = kv_launch_saa(g.nu, g.nv, g.np, d_proj, d_shifts, nplanes, kc, d_out, ctx->stream);
+    // the caller's host arrays are read by the copies: return when they are done
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         kv_set_error("shift-and-add failed: %s", hipGetErrorString(e));
         return KV_ERR_DEVICE;

@@ -182,6 +182,15 @@ hipError_t kv_launch_bp_batch(int mode, const KvProblem *probs, int nprob, long 
 hipError_t kv_launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 int kv_fp_batch_blocks_per_cu(int mode);
 int kv_bp_batch_blocks_per_cu(int mode);
+// Per-device address of the no-allocation shift-and-add staging array
+// (kv_misc.hip): _init resolves it once per device, with that device current;
+// kv_internal_scratch returns it (nullptr before _init).  Word layout:
+// [src_y: KV_SAA_NOALLOC_MAX_VIEWS][plane_z: KV_SAA_NOALLOC_MAX_PLANES]
+// [shifts: KV_SAA_NOALLOC_MAX_SHIFTS].
+hipError_t kv_internal_scratch_init(int device);
+int32_t *kv_internal_scratch(int device);
+hipError_t kv_launch_saa_shifts(int np, int nplanes, float sid, float mF, float voxel_dy, const float *src_y,
+                                const float *plane_z, int32_t *shifts, hipStream_t s);
 // Planes per chunk of the shift-and-add kernel for these shifts (shift(k, p)
 // = row shift of plane k, view p): the largest of 8, 4, 2 whose staged rows
 // fit the kernel's LDS, else 1.

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 
 #include "kv_internal.h"
 
@@ -250,7 +251,69 @@ __global__ __launch_bounds__(256) void update_kernel(int64_t n, int loop, float 
         vol[i] = loop == KV_LOOP_MLEM ? vol[i] * bp[i] : vol[i] + bp[i];
 }
 
+// Row-shift table of plane k, view p (Reconstructors.h:308-331), evaluated
+// from the staged source y positions and plane heights with the host's FP32
+// expressions (IEEE division: the library is built with correctly rounded
+// FP32 division); the host has already checked every |t| < 2e9.
+__global__ __launch_bounds__(256) void saa_shift_kernel(int np, int nplanes, float sid, float mF, float voxel_dy,
+                                                        const float *__restrict__ src_y,
+                                                        const float *__restrict__ plane_z, int32_t *__restrict__ shifts)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= np * nplanes)
+        return;
+    const int k = i / np, p = i - k * np;
+    const float mZ = sid / (sid - plane_z[k]);
+    const float coeff = mZ - mF;
+    const float world = src_y[p] * coeff;
+    shifts[i] = (int32_t)(world / voxel_dy);
+}
+
 }  // namespace
+
+// Staging of kv_shift_and_add_noalloc: the source y positions, the plane
+// heights and the shift table, in one file-scope array (each GPU holds its
+// own copy until the process ends).  Host code reaches it only through the
+// per-device address that hipGetSymbolAddress returns.
+static __device__ __align__(256) int32_t scratch[KV_SAA_NOALLOC_MAX_VIEWS + KV_SAA_NOALLOC_MAX_PLANES +
+                                                 KV_SAA_NOALLOC_MAX_SHIFTS];
+
+namespace {
+constexpr int KV_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch[KV_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t kv_internal_scratch_init(int device)
+{
+    if (device < 0 || device >= KV_MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device])
+        return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e != hipSuccess)
+        return e;
+    g_scratch[device] = (int32_t *)ptr;
+    return hipSuccess;
+}
+
+int32_t *kv_internal_scratch(int device)
+{
+    if (device < 0 || device >= KV_MAX_DEVICES)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
+
+hipError_t kv_launch_saa_shifts(int np, int nplanes, float sid, float mF, float voxel_dy, const float *src_y,
+                                const float *plane_z, int32_t *shifts, hipStream_t s)
+{
+    const int n = np * nplanes;
+    saa_shift_kernel<<<(n + 255) / 256, 256, 0, s>>>(np, nplanes, sid, mF, voxel_dy, src_y, plane_z, shifts);
+    return hipGetLastError();
+}
 
 hipError_t kv_launch_correct(int64_t n, int loop, float relax, const float *data, float *proj, hipStream_t s)
 {
