@@ -421,12 +421,29 @@ def _thresholds(runs, rmse_of, label):
     return out
 
 
+LEARNED_WEIGHTS = "cnn_schedule_to_100.npz"
+
+
 def _load_cnn_params(name):
-    with np.load(os.path.join(ROOT, "kevernals_amd", "weights", name), allow_pickle=False) as f:
+    with np.load(_weights_path(name), allow_pickle=False) as f:
         return f["params"], float(f["scale"])
 
 
-def learned_batched_runs(torch, K, G, n):
+def _weights_path(name):
+    return name if os.path.sep in name else os.path.join(ROOT, "kevernals_amd", "weights", name)
+
+
+def _learned_schedule(name):
+    """The schedule a network was trained after: its probe length (`second`,
+    stored with the weights; 8 when absent)."""
+    from kevernals_amd.trajectory import Schedule
+
+    with np.load(_weights_path(name), allow_pickle=False) as f:
+        second = int(f["second"]) if "second" in f.files else 8
+    return Schedule(second=second)
+
+
+def learned_batched_runs(torch, K, G, n, weights=LEARNED_WEIGHTS):
     """Learned plus batched at full size: n reconstructions of
     1024x1024x64 / 25 views (phantom seeds 5000+i with 200
     microcalcifications, I0 = 5e3) in one mode-A batch (persistent launches,
@@ -470,13 +487,14 @@ def learned_batched_runs(torch, K, G, n):
         refs.append(r)
     torch.cuda.synchronize()
     log(f"[bench] references: 100 batched MLEM-A iterations in {t_ref:.2f} s")
-    params, scale = _load_cnn_params("cnn_schedule_to_100.npz")
+    params, scale = _load_cnn_params(weights)
     sch = Schedule()
+    sch_l = _learned_schedule(weights)
     runs = {}
     reset()
     runs["plain"] = batched_gated_mlem(b, 100, params=None, references=refs)
     reset()
-    runs["learned"] = batched_accelerated_mlem(b, sch, params=params, scale=scale, references=refs)
+    runs["learned"] = batched_accelerated_mlem(b, sch_l, params=params, scale=scale, references=refs)
     reset()
     runs["schedule_only"] = batched_accelerated_mlem(b, sch, params=None, references=refs)
     out_t = _thresholds(runs, lambda e: (max(e[3]) if e[3] is not None else None), f"all {n}")
@@ -484,7 +502,8 @@ def learned_batched_runs(torch, K, G, n):
         "reconstructions": n, "volume": [g.nx, g.ny, g.nz], "views": g.nviews,
         "schedule": {"start": sch.start, "first": sch.first, "mid": sch.mid, "second": sch.second,
                      "horizon": sch.horizon, "mlem_iterations": sch.cost},
-        "cnn": "cnn_schedule_to_100.npz, gated, on the final jump",
+        "learned_schedule": {"second": sch_l.second, "mlem_iterations": sch_l.cost},
+        "cnn": f"{os.path.basename(weights)}, gated, on the final jump",
         "time_to_rmse": out_t,
         "gate": {"accepted": runs["learned"].accepted, "rejected": runs["learned"].rejected},
         "reference": "100 batched MLEM-A iterations, FP32",
@@ -499,7 +518,7 @@ def learned_batched_runs(torch, K, G, n):
 def bench_learned_batched(args, torch, K, G):
     """The learned plus batched workload alone (learned_batched_runs) as the
     JSON line's value: plain batched time / accelerated time to RMSE <= 1%."""
-    res, runs = learned_batched_runs(torch, K, G, args.recons)
+    res, runs = learned_batched_runs(torch, K, G, args.recons, args.cnn_weights)
     s1 = res["time_to_rmse"]["0.01"]
     out = {
         "metric": METRIC,
@@ -682,6 +701,8 @@ def main():
                     default="full")
     ap.add_argument("--recons", type=int, default=8, help="reconstructions of the learned-batched workload")
     ap.add_argument("--tile-cache", default=None, help="autotuner cache file (batched workload)")
+    ap.add_argument("--cnn-weights", default=LEARNED_WEIGHTS,
+                    help="network of the learned runs (kevernals_amd/weights/ name or a path)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an elementwise copy of 256 MiB (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -775,7 +796,7 @@ def main():
         del mu, data, x
         ctx = None
         torch.cuda.empty_cache()
-        res, _ = learned_batched_runs(torch, K, G, args.recons)
+        res, _ = learned_batched_runs(torch, K, G, args.recons, args.cnn_weights)
         s1 = res["time_to_rmse"]["0.01"]
         learned = {
             "workload": (f"learned plus batched MLEM-A: {args.recons} reconstructions 1024x1024x64 / 25 views "
