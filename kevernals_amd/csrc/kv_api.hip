@@ -105,6 +105,26 @@ bool walk_fast_ok(const kv_geometry &q)
     return true;
 }
 
+// Every z plane of the volume crosses every ray of every view at alpha >=
+// 1e-5 (evaluated in FP64 from the plane crossings (k - nz/2 - s) / d of the
+// two outer planes; the others lie between them), well above the 1e-6 the
+// backprojector's clamped weights need (kv_bp.hip, FW) whatever the FP32
+// rounding of the tables.
+bool bp_posz_ok(const kv_geometry &q)
+{
+    for (int p = 0; p < q.np; ++p) {
+        const double s = (double)(q.src[3 * p + 2] / q.sz), dz = (double)(q.det[3 * p + 2] / q.sz) - s;
+        if (!(std::fabs(dz) > 1e-20))
+            return false;
+        for (int k = 0; k <= q.nz; k += q.nz) {
+            const double a = (((double)k - (double)q.nz / 2.0) - s) / dz;
+            if (!(a >= 1e-5))
+                return false;
+        }
+    }
+    return true;
+}
+
 kv_status check_mode(int mode)
 {
     if (mode != KV_MODE_R && mode != KV_MODE_A) {
@@ -322,6 +342,7 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     g.det = ctx->d_det;
     g.src = ctx->d_src;
     g.walk_fast = walk_fast_ok(q) ? 1 : 0;
+    g.bp_posz = bp_posz_ok(q) ? 1 : 0;
     ctx->has_geom = true;
     return KV_OK;
 }

@@ -189,6 +189,14 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return __
 // the results are the FP32 ones bit for bit; when any of them is negative both
 // forms give w < 0.  No NaN reaches these operands (entries are finite or
 // +-INF).
+// a * b with 0 * anything = 0 (v_mul_legacy_f32; the IEEE product for
+// non-zero operands)
+__device__ __forceinline__ float mul_legacy(float a, float b)
+{
+    float r;
+    asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ float bmin(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
 __device__ __forceinline__ float bmax(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
 
@@ -227,7 +235,16 @@ struct BpShared {
 // Backprojection of brick (bxi, byi, bzi) of depth zt <= ZT slices.  Called
 // by all 256 threads of a workgroup (it synchronises).  SR band loads per
 // thread are issued together before their LDS writes.
-template <int MODE, int EP, int ZT, bool UNIT, int SR>
+// FW (mode R): the geometry puts every z plane at alpha >= 1e-6 on every
+// ray (KvGeom::bp_posz), so a positive weight w = H - L has L >= 1e-6 and is
+// at least one FP32 ulp of 1e-6 (~1e-13): w > 1e-24 is then w > 0, and the
+// accumulation takes the clamped weight max(w, +0) with no select (mode A,
+// whose test is w > 0, always does).  The product is a legacy multiply,
+// which gives 0 for a zero weight whatever the pixel value (a skipped
+// pixel's value may be any float, INF and NaN included, as in the
+// reference, which never reads it) and the IEEE product otherwise; adding
+// +0 leaves the sums bit-identical (they are never -0).
+template <int MODE, int EP, int ZT, bool UNIT, int SR, bool FW>
 __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, const float *__restrict__ proj,
                                          float *__restrict__ vol, const float *__restrict__ sens, int bxi, int byi,
                                          int bzi, int zt, BpShared<ZT> &sm)
@@ -504,20 +521,25 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                         for (int c = 0; c < NC; ++c) {
                             const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
                             const bool ok = w > tol;
-                            // selects, not adds of zero: skipped pixels leave
-                            // the running sums bit-identical
+                            // selects, or (FW, mode A) the clamped weight:
+                            // skipped pixels leave the running sums
+                            // bit-identical either way
+                            const float wp = bmax(w, 0.f);
                             if (EP == KV_EP_COUNT) {
                                 wsm = ok ? wsm + 1.f : wsm;
+                            } else if (MODE == KV_MODE_R && FW) {
+                                wsm = wsm + wp;
+                                a = a + mul_legacy(wp, v[c]);
                             } else if (MODE == KV_MODE_R) {
                                 wsm = ok ? wsm + w : wsm;
                                 a = ok ? a + w * v[c] : a;
                             } else if (UNIT) {
                                 const float L = sqrtf((L2[c] + lyy) + lz2);
-                                a = ok ? a + w * L : a;
+                                a = a + wp * L;
                             } else {
                                 // input already multiplied by the ray's
                                 // length factor (kv_launch_scale_a)
-                                a = ok ? a + w * v[c] : a;
+                                a = a + mul_legacy(wp, v[c]);
                             }
                         }
                     };
@@ -629,15 +651,15 @@ __global__ __launch_bounds__(256) void scale_a_kernel(KvGeom g, KvTables t, cons
 }
 
 // One brick per workgroup.
-template <int MODE, int EP, int ZT, bool UNIT>
+template <int MODE, int EP, int ZT, bool UNIT, bool FW>
 __global__ __launch_bounds__(256, 5) void bp_tiled_kernel(KvGeom g, KvTables t, const float *__restrict__ proj,
                                                        float *__restrict__ vol, const float *__restrict__ sens)
 {
     __shared__ BpShared<ZT> sm;
     // rounds of 3 loads: mode A measured 3 % faster; mode R's registers would
     // spill with them
-    bp_brick<MODE, EP, ZT, UNIT, MODE == KV_MODE_A ? 3 : 1>(g, t, proj, vol, sens, blockIdx.x, blockIdx.y,
-                                                           blockIdx.z, ZT, sm);
+    bp_brick<MODE, EP, ZT, UNIT, MODE == KV_MODE_A ? 3 : 1, FW>(g, t, proj, vol, sens, blockIdx.x, blockIdx.y,
+                                                               blockIdx.z, ZT, sm);
 }
 
 // Persistent backprojection + MLEM update over a batch of reconstructions:
@@ -668,7 +690,8 @@ __global__ __launch_bounds__(256, 5) void bp_batch_kernel(const KvProblem *__res
         const int bz = (int)(local / per_z);
         const int rem = (int)(local - (long long)bz * per_z);
         const int by = rem / P.bp_bx, bx = rem - by * P.bp_bx;
-        bp_brick<MODE, KV_EP_MLEM, kZT, false, 1>(P.g, P.t, P.ratio, P.vol, P.sens, bx, by, bz, P.bp_zt, sm);
+        bp_brick<MODE, KV_EP_MLEM, kZT, false, 1, false>(P.g, P.t, P.ratio, P.vol, P.sens, bx, by, bz, P.bp_zt,
+                                                         sm);
     }
 }
 
@@ -684,7 +707,10 @@ hipError_t launch_tiled(const KvGeom &g, const KvTables &t, const float *proj, f
                         hipStream_t s)
 {
     const dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kZT - 1) / kZT), block(256);
-    bp_tiled_kernel<MODE, EP, kZT, UNIT><<<grid, block, 0, s>>>(g, t, proj, vol, sens);
+    if (MODE == KV_MODE_R && (EP == KV_EP_MLEM || EP == KV_EP_SART || EP == KV_EP_PLAIN) && g.bp_posz)
+        bp_tiled_kernel<MODE, EP, kZT, UNIT, true><<<grid, block, 0, s>>>(g, t, proj, vol, sens);
+    else
+        bp_tiled_kernel<MODE, EP, kZT, UNIT, false><<<grid, block, 0, s>>>(g, t, proj, vol, sens);
     return hipGetLastError();
 }
 

@@ -312,16 +312,26 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
     // bounds-checked buffer resource over the volume (an offset outside it
     // reads 0, and a rejected segment's value is never used), and rejected
     // segments leave the sums untouched (selects, not additions of zero).
+    // Each segment is added one step after its gather is issued, so the
+    // load is in flight while the next segment is walked; the sums take
+    // the segments in the same order.
     const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
+    float pw = 0.f, pv = 0.f;
+    bool pok = false;
     auto acc = [&](bool ok, unsigned k, float w) {
         const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
-        tw = ok ? tw + w : tw;
-        tot = ok ? tot + w * val : tot;
+        tw = pok ? tw + pw : tw;
+        tot = pok ? tot + pw * pv : tot;
+        pok = ok;
+        pw = w;
+        pv = val;
     };
     if (g.walk_fast)
         walk_r<true>(g, r, acc);
     else
         walk_r<false>(g, r, acc);
+    tw = pok ? tw + pw : tw;
+    tot = pok ? tot + pw * pv : tot;
     const float fp = tw > 1e-6f ? tot / tw : 0.f;
     if (EP == KV_EP_PLAIN) {
         out[o] = fp;

@@ -20,6 +20,7 @@ struct KvGeom {
     const float *det;
     const float *src;
     int walk_fast;  // forward walk may use end thresholds and 24-bit indices (see kv_fp.hip)
+    int bp_posz;    // every view's z planes at alpha >= 1e-6: clamped backprojection weights (see kv_bp.hip)
 };
 
 // Per-view plane-crossing tables.  For mode R an entry is the reference's

@@ -448,3 +448,40 @@ def test_host_variants_match_device(K):
     bp = torch.empty(g.vol_shape, device="cuda")
     ctx.back_project(dev(y), bp, K.MODE_R)
     np.testing.assert_array_equal(ctx.back_project_host(y), host(bp))
+
+
+def test_bp_r_skipped_pixels_ignore_non_finite_values(K, O):
+    """Pixels whose weight test fails are never read by the reference
+    (Projector.cu:710-729), so INF and NaN there must not reach the sums;
+    those of accepted pixels must (INF / NaN voxels, as the oracle).  Run on
+    a geometry with the clamped-weight backprojection (every z plane at
+    alpha >= 1e-5 along every view) and on one without it (the sources
+    1e-4 above the top plane), mode R bit for bit, mode A to the oracle's
+    tolerance."""
+    from kevernals_amd.geometry import Geometry
+
+    g_fast = half_small()
+    det = np.array([[-20.0, -20.0, -30.0]] * 3, np.float32)
+    src = np.array([[0.0, -3.0, 8.0001], [0.0, 0.0, 8.0001], [0.0, 3.0, 8.0001]], np.float32)  # top plane z = 8
+    g_slow = Geometry(16, 16, 8, 1.0, 1.0, 2.0, 40, 40, 3, 1.0, 1.0, det, src)
+    for g in (g_fast, g_slow):
+        o = O.Oracle(g)
+        proj = (0.5 + rng(51).random(g.proj_shape)).astype(np.float32)
+        r = rng(52)
+        flat = proj.reshape(-1)
+        idx = r.choice(flat.size, size=flat.size // 50, replace=False)
+        flat[idx[0::3]] = np.inf
+        flat[idx[1::3]] = -np.inf
+        flat[idx[2::3]] = np.nan
+        ctx = K.Context(0, g)
+        out = torch.empty(g.vol_shape, device="cuda")
+        ctx.back_project(dev(proj), out, K.MODE_R)
+        got, ref = host(out), o.bp_r(proj)
+        np.testing.assert_array_equal(got, ref)  # NaN equal to NaN
+        assert np.isfinite(ref).mean() > 0.2 and (~np.isfinite(ref)).any()
+        ctx.back_project(dev(proj), out, K.MODE_A)
+        got, ref = host(out), o.bp_a(proj)
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_array_equal(np.isinf(got), np.isinf(ref))
+        fin = np.isfinite(ref)
+        np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-5, atol=1e-6)
