@@ -189,14 +189,6 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return __
 // the results are the FP32 ones bit for bit; when any of them is negative both
 // forms give w < 0.  No NaN reaches these operands (entries are finite or
 // +-INF).
-// a * b with 0 * anything = 0 (v_mul_legacy_f32; the IEEE product for
-// non-zero operands)
-__device__ __forceinline__ float mul_legacy(float a, float b)
-{
-    float r;
-    asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 __device__ __forceinline__ float bmin(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
 __device__ __forceinline__ float bmax(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
 
@@ -529,7 +521,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                                 wsm = ok ? wsm + 1.f : wsm;
                             } else if (MODE == KV_MODE_R && FW) {
                                 wsm = wsm + wp;
-                                a = a + mul_legacy(wp, v[c]);
+                                a = a + kv_mul_legacy(wp, v[c]);
                             } else if (MODE == KV_MODE_R) {
                                 wsm = ok ? wsm + w : wsm;
                                 a = ok ? a + w * v[c] : a;
@@ -539,7 +531,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             } else {
                                 // input already multiplied by the ray's
                                 // length factor (kv_launch_scale_a)
-                                a = a + mul_legacy(wp, v[c]);
+                                a = a + kv_mul_legacy(wp, v[c]);
                             }
                         }
                     };

@@ -245,11 +245,39 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         prev = fill;
         filler_left = false;
     }
-    advance(prev);
     // entries below the fillers' value (all of them when there is no filler);
     // two entries per turn, so that the previous entry needs no copy
     const float lim = filler_left ? fill : kInf;
-    if (FAST) {
+    // The entries below lim are known in advance when every run of two or
+    // more entries ends below lim (its values lie below its end threshold):
+    // a run of one entry contributes it if it lies below lim.  Each step of
+    // advance1 then takes exactly one of them, so the loop below runs a fixed
+    // count of steps: those entries, less the ones the first entry takes
+    // (every run head equal to it).  Run entries at or above lim stay for
+    // the part after the fillers.
+    auto ends_below = [&](int c, float t) { return c <= 1 || t < lim; };
+    auto below = [&](int c, float a) { return c == 1 ? (int)(a < lim) : c; };
+    const bool counted = FAST && ends_below(cx, tx) && ends_below(cy, ty) && ends_below(cz, tz);
+    const int left = (below(cx, ax) + below(cy, ay) + below(cz, az)) - ((ax == prev) + (ay == prev) + (az == prev));
+    advance(prev);
+    if (counted) {
+        int k = left;
+        for (; k >= 2; k -= 2) {
+            const float c1 = min3f(ax, ay, az);
+            advance1(c1);
+            segment(prev, c1);
+            const float c2 = min3f(ax, ay, az);
+            advance1(c2);
+            segment(c1, c2);
+            prev = c2;
+        }
+        if (k > 0) {
+            const float c1 = min3f(ax, ay, az);
+            advance1(c1);
+            segment(prev, c1);
+            prev = c1;
+        }
+    } else if (FAST) {
         for (;;) {
             const float c1 = min3f(ax, ay, az);
             if (!(c1 < lim))
@@ -310,28 +338,27 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
     float tot = 0.f, tw = 0.f;
     // Branch-free accumulation: every segment gathers through a
     // bounds-checked buffer resource over the volume (an offset outside it
-    // reads 0, and a rejected segment's value is never used), and rejected
-    // segments leave the sums untouched (selects, not additions of zero).
-    // Each segment is added one step after its gather is issued, so the
+    // reads 0; a rejected segment's index may lie outside it).  Each segment is added one step after its gather is issued, so the
     // load is in flight while the next segment is walked; the sums take
-    // the segments in the same order.
+    // the segments in the same order.  A rejected segment carries weight +0
+    // and adds +0 to tw and kv_mul_legacy(+0, value) = +-0 to tot (for any
+    // value, INF and NaN included); neither sum is ever -0 (they start at
+    // +0 and x + (-x) rounds to +0), so both are left bit-identical.
     const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
     float pw = 0.f, pv = 0.f;
-    bool pok = false;
     auto acc = [&](bool ok, unsigned k, float w) {
         const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
-        tw = pok ? tw + pw : tw;
-        tot = pok ? tot + pw * pv : tot;
-        pok = ok;
-        pw = w;
+        tw = tw + pw;
+        tot = tot + kv_mul_legacy(pw, pv);
+        pw = ok ? w : 0.f;
         pv = val;
     };
     if (g.walk_fast)
         walk_r<true>(g, r, acc);
     else
         walk_r<false>(g, r, acc);
-    tw = pok ? tw + pw : tw;
-    tot = pok ? tot + pw * pv : tot;
+    tw = tw + pw;
+    tot = tot + kv_mul_legacy(pw, pv);
     const float fp = tw > 1e-6f ? tot / tw : 0.f;
     if (EP == KV_EP_PLAIN) {
         out[o] = fp;

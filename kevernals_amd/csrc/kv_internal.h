@@ -117,6 +117,16 @@ kv_status kv_internal_ensure_sens(kv_context *ctx);
         }                                                                                       \
     } while (0)
 
+// a * b with 0 * anything = +-0, also for INF and NaN (v_mul_legacy_f32); the
+// IEEE product when neither operand is zero.  A sum s + kv_mul_legacy(0, x)
+// is s bit for bit unless s is -0.
+__device__ __forceinline__ float kv_mul_legacy(float a, float b)
+{
+    float r;
+    asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // AlmostEqualRelative(a, 0.F) of commons/Maths.h:25-34: the projection
 // values MLEM leaves unchanged (Reconstructors.h:207-213).
 __device__ __forceinline__ bool almost_equal_zero(float a)

@@ -485,3 +485,30 @@ def test_bp_r_skipped_pixels_ignore_non_finite_values(K, O):
         np.testing.assert_array_equal(np.isinf(got), np.isinf(ref))
         fin = np.isfinite(ref)
         np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-5, atol=1e-6)
+
+
+def test_fp_non_finite_voxels_reach_only_rays_through_them(K, O):
+    """INF and NaN voxels enter the ray sums of the rays whose accepted
+    segments cross them (Projector.cu:356-366) and no others; mode R bit for
+    bit (NaN equal to NaN), mode A with the same non-finite pattern as the
+    oracle and its tolerance elsewhere."""
+    g = half_small()
+    o = O.Oracle(g)
+    x = (0.5 + rng(53).random(g.vol_shape)).astype(np.float32)
+    flat = x.reshape(-1)
+    idx = rng(54).choice(flat.size, size=30, replace=False)
+    flat[idx[0::3]] = np.inf
+    flat[idx[1::3]] = -np.inf
+    flat[idx[2::3]] = np.nan
+    ctx = K.Context(0, g)
+    out = torch.empty(g.proj_shape, device="cuda")
+    ctx.forward_project(dev(x), out, K.MODE_R)
+    got, ref = host(out), o.fp_r(x)
+    np.testing.assert_array_equal(got, ref)
+    assert 0.001 < (~np.isfinite(ref)).mean() < 0.5
+    ctx.forward_project(dev(x), out, K.MODE_A)
+    got, ref = host(out), o.fp_a(x)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    np.testing.assert_array_equal(np.isinf(got), np.isinf(ref))
+    fin = np.isfinite(ref)
+    np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-5, atol=1e-5)
