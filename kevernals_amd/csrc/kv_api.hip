@@ -125,6 +125,34 @@ bool bp_posz_ok(const kv_geometry &q)
     return true;
 }
 
+// KvGeom::rc on the host: the same FP32 operations, in the same order, as
+// the per-ray set-up of the forward kernels (Projector.cu:303-319 and the
+// z terms of :54-61 and :186), so every value is bit-identical.
+std::vector<float> ray_constants(const kv_geometry &q)
+{
+    std::vector<float> rc((size_t)q.np * (KV_RC + q.nu + q.nv), 0.f);
+    float *qx = rc.data() + (size_t)q.np * KV_RC, *qy = qx + (size_t)q.np * q.nu;
+    const float hz = (float)q.nz / 2.f;
+    for (int p = 0; p < q.np; ++p) {
+        const float *S = q.src + 3 * p, *D = q.det + 3 * p;
+        const float sz = S[2] / q.sz, dz = D[2] / q.sz - sz;
+        const bool okz = std::fabs(dz) > 1e-6f;
+        float *c = rc.data() + (size_t)KV_RC * p;
+        c[0] = S[0] / q.sx;
+        c[1] = S[1] / q.sy;
+        c[2] = sz;
+        c[3] = dz;
+        c[4] = okz ? (-hz - sz) / dz : 0.f;
+        c[5] = okz ? (hz - sz) / dz : 0.f;
+        c[6] = okz ? 1.f / dz : 0.f;
+        for (int u = 0; u < q.nu; ++u)
+            qx[(size_t)p * q.nu + u] = (D[0] + ((float)u + 0.5f) * q.pu) / q.sx;
+        for (int v = 0; v < q.nv; ++v)
+            qy[(size_t)p * q.nv + v] = (D[1] + ((float)v + 0.5f) * q.pv) / q.sy;
+    }
+    return rc;
+}
+
 kv_status check_mode(int mode)
 {
     if (mode != KV_MODE_R && mode != KV_MODE_A) {
@@ -187,6 +215,7 @@ void release_geometry(kv_context *ctx)
 {
     free_dev(ctx->d_det);
     free_dev(ctx->d_src);
+    free_dev(ctx->d_rc);
     free_tables(ctx->tab_r);
     free_tables(ctx->tab_a);
     free_dev(ctx->d_ratio);
@@ -328,12 +357,17 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     kv_status st;
     if ((st = dev_alloc((void **)&ctx->d_det, (size_t)q.np * 12)) ||
         (st = dev_alloc((void **)&ctx->d_src, (size_t)q.np * 12)) ||
+        (st = dev_alloc((void **)&ctx->d_rc, (size_t)q.np * (KV_RC + q.nu + q.nv) * 4)) ||
         (st = dev_alloc((void **)&ctx->d_ratio, (size_t)nray * 4))) {
         release_geometry(ctx);
         return st;
     }
     KV_TRY_HIP(hipMemcpy(ctx->d_det, q.det, (size_t)q.np * 12, hipMemcpyHostToDevice));
     KV_TRY_HIP(hipMemcpy(ctx->d_src, q.src, (size_t)q.np * 12, hipMemcpyHostToDevice));
+    {
+        const std::vector<float> rc = ray_constants(q);
+        KV_TRY_HIP(hipMemcpy(ctx->d_rc, rc.data(), rc.size() * 4, hipMemcpyHostToDevice));
+    }
     KvGeom &g = ctx->g;
     g.nx = q.nx; g.ny = q.ny; g.nz = q.nz;
     g.nu = q.nu; g.nv = q.nv; g.np = q.np;
@@ -341,6 +375,7 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     g.pu = q.pu; g.pv = q.pv;
     g.det = ctx->d_det;
     g.src = ctx->d_src;
+    g.rc = ctx->d_rc;
     g.walk_fast = walk_fast_ok(q) ? 1 : 0;
     g.bp_posz = bp_posz_ok(q) ? 1 : 0;
     ctx->has_geom = true;

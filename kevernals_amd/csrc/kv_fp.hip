@@ -41,28 +41,32 @@ __device__ __forceinline__ bool feq(float a, float b, float t)
 __device__ __forceinline__ bool fsupeq(float a, float b, float t) { return a > b || feq(a, b, t); }
 
 struct RayF {
-    float sx, sy, sz;  // source, floating-voxel units
-    float dx, dy, dz;  // source -> pixel centre
-    float hx, hy, hz;  // half sizes
+    float sx, sy, sz;    // source, floating-voxel units
+    float dx, dy, dz;    // source -> pixel centre
+    float hx, hy, hz;    // half sizes
+    float a1z, aNz, stz; // z-face alphas and 1/dz (0 when |dz| <= 1e-6)
 };
 
-// Projector.cu:303-323
+// Projector.cu:303-323.  The terms that depend on the view alone, and the
+// pixel-centre coordinates, come from the geometry's ray constants (the same
+// FP32 operations evaluated once on the host).
 __device__ __forceinline__ void ray_setup(const KvGeom &g, int p, int u, int v, RayF &r)
 {
-    const float *S = g.src + 3 * p;
-    const float *D = g.det + 3 * p;
-    r.sx = S[0] / g.sx;
-    r.sy = S[1] / g.sy;
-    r.sz = S[2] / g.sz;
-    const float qx = (D[0] + ((float)u + 0.5f) * g.pu) / g.sx;
-    const float qy = (D[1] + ((float)v + 0.5f) * g.pv) / g.sy;
-    const float qz = D[2] / g.sz;
+    const float *C = g.rc + KV_RC * p;
+    r.sx = C[0];
+    r.sy = C[1];
+    r.sz = C[2];
+    const float qx = g.rc[(size_t)g.np * KV_RC + (size_t)p * g.nu + u];
+    const float qy = g.rc[(size_t)g.np * (KV_RC + g.nu) + (size_t)p * g.nv + v];
     r.dx = qx - r.sx;
     r.dy = qy - r.sy;
-    r.dz = qz - r.sz;
+    r.dz = C[3];
     r.hx = (float)g.nx / 2.f;
     r.hy = (float)g.ny / 2.f;
     r.hz = (float)g.nz / 2.f;
+    r.a1z = C[4];
+    r.aNz = C[5];
+    r.stz = C[6];
 }
 
 // Projector.cu:88-120
@@ -119,10 +123,10 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     const bool okx = fabsf(r.dx) > tol, oky = fabsf(r.dy) > tol, okz = fabsf(r.dz) > tol;
     const float a1x = okx ? (-r.hx - r.sx) / r.dx : 0.f;
     const float a1y = oky ? (-r.hy - r.sy) / r.dy : 0.f;
-    const float a1z = okz ? (-r.hz - r.sz) / r.dz : 0.f;
+    const float a1z = r.a1z;  // okz ? (-hz - sz) / dz : 0
     const float aNx = okx ? (r.hx - r.sx) / r.dx : 0.f;
     const float aNy = oky ? (r.hy - r.sy) / r.dy : 0.f;
-    const float aNz = okz ? (r.hz - r.sz) / r.dz : 0.f;
+    const float aNz = r.aNz;
     const float amin = fmaxf(fmaxf(0.f, fminf(a1x, aNx)), fmaxf(fminf(a1y, aNy), fminf(a1z, aNz)));
     const float amax = fminf(fminf(1.f, fmaxf(a1x, aNx)), fminf(fmaxf(a1y, aNy), fmaxf(a1z, aNz)));
     if (fsupeq(amin, amax, 1e-7f))
@@ -153,7 +157,7 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     if (okz) {
         // z run written over (hiz-loz)-1 slots, at least its first entry
         // (Projector.cu:181-201).
-        const float st = 1.f / r.dz;
+        const float st = r.stz;  // 1 / dz
         az = st * (-r.hz - r.sz + (float)(r.dz > 0 ? loz : hiz));
         iz = fabsf(st);
         const int dzr = hiz - loz;

@@ -19,9 +19,18 @@ struct KvGeom {
     float pu, pv;
     const float *det;
     const float *src;
+    // Ray constants in FP32, evaluated by kv_set_geometry as the forward
+    // kernels would: rc[KV_RC*p + 0..6] = source x, y, z over the spacing,
+    // dz (detector z / sz - source z / sz), the z-face alphas
+    // (-nz/2 - sz_p) / dz and (nz/2 - sz_p) / dz and 1 / dz (the last three
+    // 0 when |dz| <= 1e-6); then qx[p][u] = (det x + (u + 0.5) pu) / sx at
+    // rc + KV_RC*np and qy[p][v] after it.
+    const float *rc;
     int walk_fast;  // forward walk may use end thresholds and 24-bit indices (see kv_fp.hip)
     int bp_posz;    // every view's z planes at alpha >= 1e-6: clamped backprojection weights (see kv_bp.hip)
 };
+
+constexpr int KV_RC = 8;
 
 // Per-view plane-crossing tables.  For mode R an entry is the reference's
 // alpha1/alpha2 of Projector.cu:653-669 for plane k of a pixel's ray:
@@ -83,6 +92,7 @@ struct kv_context {
     KvGeom g{};
     float *d_det = nullptr;
     float *d_src = nullptr;
+    float *d_rc = nullptr;     // KvGeom::rc
     KvTables tab_r{};
     bool tab_r_ok = false;
     KvTables tab_a{};
