@@ -391,21 +391,26 @@ __device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int ti
 // Forward tile of this workgroup in an XCD-aware order.  Workgroups are
 // handed to the eight XCDs round-robin by linear id.  The launch is
 // one-dimensional over gxp x gy x np tiles; tile columns are grouped in
-// stripes of `chunk` tiles, stripe i going to XCD i % 8, and each XCD walks
-// its stripes column-fastest, then rows, then views.  A stripe keeps an XCD's
-// L2 on a few neighbouring detector columns (round-robin single tiles of an
-// 8-column layout made every L2 fetch the same volume lines, 43.5 GB per
-// full-size launch), and the interleaving keeps the eight XCDs' shares of
-// the (uneven) ray lengths balanced (one contiguous band per XCD left the
-// outer bands idle: 11.4 ms against 8.8 ms).  gxp = tile columns rounded up
-// to a multiple of 8 x chunk; tiles of padding columns (u past the detector)
-// exit at the kernel's bounds test.  A launch covers views p0, p0 + 1, ...
+// stripes of `chunk` tiles, and each XCD walks its stripes column-fastest,
+// then rows, then views.  In tile row r (rows and views counted together)
+// stripe i goes to XCD (i - r) mod 8.  A stripe keeps an XCD's L2 on a few
+// neighbouring detector columns (round-robin single tiles of an 8-column
+// layout made every L2 fetch the same volume lines, 43.5 GB per full-size
+// launch), and the interleaving balances the eight XCDs' shares of the
+// uneven ray lengths (one contiguous band per XCD left the outer bands
+// idle: 11.4 ms against 8.8 ms).  Rotating the assignment by one stripe per
+// tile row gives every XCD every stripe equally often: with a fixed
+// i mod 8 two XCDs carried 7 % more than the mean at full size, as the
+// detector's columns hold ray lengths in a V profile cut off at the edges
+// of the volume's shadow.  gxp = tile columns rounded up to a multiple of
+// 8 x chunk; tiles of padding columns (u past the detector) exit at the
+// kernel's bounds test.  A launch covers views p0, p0 + 1, ...
 __device__ __forceinline__ void fp_xcd_tile(int gxp, int gy, int chunk, int p0, int &tx, int &ty, int &p)
 {
     constexpr unsigned X = KV_XCDS;
     const unsigned w = blockIdx.x, c = (unsigned)chunk, nch = (unsigned)gxp / (X * c);
     const unsigned j = w / X, t = j % c, jc = j / c, ci = jc % nch, r = jc / nch;
-    tx = (int)((ci * X + w % X) * c + t);
+    tx = (int)((ci * X + (w + r) % X) * c + t);
     ty = (int)(r % (unsigned)gy);
     p = p0 + (int)(r / (unsigned)gy);
 }
