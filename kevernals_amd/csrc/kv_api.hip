@@ -38,6 +38,7 @@ void free_tables(KvTables &t)
     free_dev(t.ly2);
     free_dev(t.lz2);
     free_dev(t.zs);
+    free_dev(t.yz);
 }
 
 kv_status dev_alloc(void **p, size_t bytes)
@@ -145,6 +146,8 @@ std::vector<float> ray_constants(const kv_geometry &q)
         c[4] = okz ? (-hz - sz) / dz : 0.f;
         c[5] = okz ? (hz - sz) / dz : 0.f;
         c[6] = okz ? 1.f / dz : 0.f;
+        c[8] = D[0] / q.sx;
+        c[9] = (D[0] + (float)q.nu * q.pu) / q.sx;
         for (int u = 0; u < q.nu; ++u)
             qx[(size_t)p * q.nu + u] = (D[0] + ((float)u + 0.5f) * q.pu) / q.sx;
         for (int v = 0; v < q.nv; ++v)
@@ -166,14 +169,15 @@ kv_status ensure_tables(kv_context *ctx, int mode)
 {
     const KvGeom &g = ctx->g;
     const size_t nxe = (size_t)g.np * g.nu * (g.nx + 1), nye = (size_t)g.np * g.nv * (g.ny + 1),
-                 nze = (size_t)g.np * (g.nz + 1);
+                 nze = (size_t)g.np * (g.nz + 1), nyz = (size_t)g.np * g.nz * g.ny * KV_YZ;
     if (mode == KV_MODE_R) {
         if (ctx->tab_r_ok)
             return KV_OK;
         KvTables &t = ctx->tab_r;
         kv_status st;
         if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
-            (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4))) {
+            (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4)) ||
+            (st = dev_alloc((void **)&t.yz, nyz * 4))) {
             free_tables(t);
             return st;
         }
@@ -188,7 +192,8 @@ kv_status ensure_tables(kv_context *ctx, int mode)
     if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
         (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.lx2, (size_t)g.np * g.nu * 4)) ||
         (st = dev_alloc((void **)&t.ly2, (size_t)g.np * g.nv * 4)) ||
-        (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4))) {
+        (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4)) ||
+        (st = dev_alloc((void **)&t.yz, nyz * 4))) {
         free_tables(t);
         return st;
     }

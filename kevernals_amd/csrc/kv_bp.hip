@@ -13,19 +13,22 @@
 //  * A pixel contributes only when its ray crosses the voxel box with positive
 //    alpha-length (weight > 1e-24, Projector.cu:710-725), i.e. when the pixel
 //    centre lies inside the box's shadow on the detector.  Each lane loops
-//    over the window intersected with that shadow (widened by 0.01 pixel);
+//    over the window intersected with that shadow (widened by 0.02 pixel);
 //    the skipped pixels are exactly those whose weight test fails, and the
 //    accumulation order of the others is unchanged.
-//  * The detector row of a voxel's shadow and window depends only on (y, z),
+//  * The detector rows of a voxel's shadow and window depend only on (y, z),
 //    so the 64 lanes of a wavefront (64 consecutive x of one (y, z) row) walk
-//    the same rows: the y and z terms are wave-uniform.
+//    the same rows: the y and z terms are wave-uniform.  They are tabulated
+//    once per geometry (KvTables::yz) and, with the y-table rows and the
+//    per-view constants, read by scalar loads, so the scalar unit does that
+//    share of the work beside the vector unit.
 //
 // A 256-thread workgroup owns a 64 x 4 x ZT brick (one wavefront per y row,
-// looping over ZT slices).  Per view it stages the bands of the x and y
-// tables its brick needs in LDS; the x band has an odd row stride so that
-// the lanes' reads, which sit at similar offsets of consecutive rows, fall in
-// different banks.  Table entries outside a band are read from the global
-// table, so the band widths only affect speed.
+// looping over ZT slices).  Per view it stages the bands of the x table its
+// brick needs in LDS, with an odd row stride so that the lanes' reads, which
+// sit at similar offsets of consecutive rows, fall in different banks.  Table
+// entries outside a band are read from the global table, so the band width
+// only affects speed.
 #pragma clang fp contract(off)
 
 #include "kv_internal.h"
@@ -176,7 +179,6 @@ constexpr int TX = 64;           // voxels along x per workgroup (one per lane)
 constexpr int TY = 4;            // y rows per workgroup (one per wavefront)
 constexpr int XBW = 32;          // x band width (detector columns per plane)
 constexpr int XBS = XBW + 1;     // its LDS row stride (odd: conflict-free)
-constexpr int YBR = 96;          // y band rows
 
 // lo < hi and v not NaN: the median of the three is the clamp
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
@@ -213,16 +215,102 @@ __device__ __forceinline__ void shadow_affine(float c, float k1a, float k0a, flo
     hi = min((int)floorf(b), n - 1);
 }
 
-// LDS of one brick: table bands and the per-voxel running sums.
+// Per view, z slice and y row: the terms of the backprojector that the
+// voxels of one (y, z) row share (KvTables::yz), with the expressions the
+// backprojector evaluated per row before they were tabulated, so every value
+// is the same: the detector-row range of the voxel box's shadow between the
+// slice's planes (shadow_affine on the zs table), in mode R intersected with
+// the +-5 window around the row of the voxel centre's projection and empty
+// when that centre lies off the detector widened by 5 pixels
+// (Projector.cu:473-492, 607-646), and the z-plane alpha interval.
+template <int MODE>
+__global__ void yz_kernel(KvGeom g, KvTables t)
+{
+    const int64_t n = (int64_t)g.np * g.nz * g.ny;
+    const int nz1 = g.nz + 1;
+    const float hy = (float)g.ny / 2.f;
+    const int half = 5;
+    const float wy = (float)half * g.pv / g.sy;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i % g.ny);
+        const int64_t r = i / g.ny;
+        const int z = (int)(r % g.nz), p = (int)(r / g.nz);
+        const float *S = g.src + 3 * p;
+        const float *D = g.det + 3 * p;
+        const float sfy = S[1] / g.sy, ofy = D[1] / g.sy;
+        const float cvy = (float)y - hy;  // Projector.cu:437-440
+        const float rdy = (cvy + 0.5f) - sfy;
+        const float efy = (D[1] + (float)g.nv * g.pv) / g.sy;
+        const float *Z = t.zs + ((size_t)p * nz1 + z) * KV_ZS;
+        const float t0 = Z[0], t1 = Z[KV_ZS];
+        const bool hs = t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f;
+        int f0 = 0, f1 = g.nv - 1;
+        if (hs)
+            shadow_affine(cvy, Z[4], Z[5], Z[KV_ZS + 4], Z[KV_ZS + 5], g.nv, f0, f1);
+        if (MODE == KV_MODE_R) {
+            const float iy = sfy + rdy * Z[1];
+            const bool yin = iy > ofy - wy && iy < efy + wy;
+            const int pcy = (int)(((iy - ofy) * g.sy) / g.pv);
+            int wly = pcy - half, why = pcy + half;
+            wly = wly < 0 ? 0 : (wly >= g.nv ? g.nv - 1 : wly);
+            why = why < 0 ? 0 : (why >= g.nv ? g.nv - 1 : why);
+            f0 = max(f0, wly);
+            f1 = min(f1, why);
+            if (!yin) {
+                f0 = 0;
+                f1 = -1;
+            }
+        }
+        const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
+        float *o = t.yz + i * KV_YZ;
+        o[0] = __int_as_float(f0);
+        o[1] = __int_as_float(f1);
+        o[2] = __int_as_float(hs ? 1 : 0);
+        o[3] = 0.f;
+        o[4] = fmaxf(0.f, fminf(a1z, a2z));
+        o[5] = fminf(1.f, fmaxf(a1z, a2z));
+        o[6] = bmax(0.f, fminf(a1z, a2z));  // lz with a +0 floor in bit order
+        o[7] = 0.f;
+    }
+}
+
+// LDS of one brick: the x-table bands and the per-voxel running sums.
 template <int ZT>
 struct BpShared {
     float xb[(TX + 1) * XBS];
     int xbase[TX + 1];
-    float yb[(TY + 1) * YBR];
-    int ybase_s;
     float accs[ZT][2][256];  // (value, weight) per slice of the brick
-    float zsb[(ZT + 1) * KV_ZS];  // per-view table rows of planes z0 .. z0 + ZT
 };
+
+// Read-only device memory seen through the constant address space: loads at
+// wavefront-uniform addresses become scalar loads (and scalar arithmetic).
+// Only for memory no kernel running at the same time writes.
+typedef const __attribute__((address_space(4))) float KvConstF;
+__device__ __forceinline__ KvConstF *kv_const(const float *p) { return (KvConstF *)(size_t)p; }
+
+// IEEE min/max of two floats that are results of arithmetic (never
+// signalling NaNs), without the operand quieting fminf/fmaxf add.
+__device__ __forceinline__ float vmin(float a, float b)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float vmax(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// floor(x) as an integer in one instruction; x must lie well inside the int
+// range (callers clamp it first).
+__device__ __forceinline__ int cvt_flr(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 
 // Backprojection of brick (bxi, byi, bzi) of depth zt <= ZT slices.  Called
 // by all 256 threads of a workgroup (it synchronises).  SR band loads per
@@ -235,7 +323,9 @@ struct BpShared {
 // which gives 0 for a zero weight whatever the pixel value (a skipped
 // pixel's value may be any float, INF and NaN included, as in the
 // reference, which never reads it) and the IEEE product otherwise; adding
-// +0 leaves the sums bit-identical (they are never -0).
+// +0 leaves the sums bit-identical (they are never -0).  The clamp is the
+// subtraction's own [0, 1] output clamp: H <= 1 and L >= +0, so w <= 1 and
+// clamping to [0, 1] is max(w, +0).
 template <int MODE, int EP, int ZT, bool UNIT, int SR, bool FW>
 __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, const float *__restrict__ proj,
                                          float *__restrict__ vol, const float *__restrict__ sens, int bxi, int byi,
@@ -243,37 +333,40 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
 {
     float *xb = sm.xb;
     int *xbase = sm.xbase;
-    float *yb = sm.yb;
-    int &ybase_s = sm.ybase_s;
     auto &accs = sm.accs;
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
+    // the wavefront index is uniform: its y row, table rows and detector
+    // rows are scalar values
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = bxi * TX, y0 = byi * TY, z0 = bzi * zt;
     const int x = x0 + lane, y = y0 + wv;
     const int nzt = min(zt, g.nz - z0);
     const bool active = x < g.nx && y < g.ny;
-    const float hx = (float)g.nx / 2.f, hy = (float)g.ny / 2.f, hz = (float)g.nz / 2.f;
-    const float cvx = (float)x - hx, cvy = (float)y - hy;  // Projector.cu:437-440
+    const float hx = (float)g.nx / 2.f;
+    const float cvx = (float)x - hx;  // Projector.cu:437-440
     const int nx1 = g.nx + 1, ny1 = g.ny + 1, nz1 = g.nz + 1;
-    const float scx = g.sx / g.pu, scy = g.sy / g.pv;
+    const float scx = g.sx / g.pu;
     const float tol = MODE == KV_MODE_R ? 1e-24f : 0.f;
     const int half = 5;
-    const float wx = (float)half * g.pu / g.sx, wy = (float)half * g.pv / g.sy;
+    const float wx = (float)half * g.pu / g.sx;
     const float ipu = 1.f / g.pu;
     const bool fast_window = g.nu < (1 << 22) && g.nv < (1 << 22);
+    const float pad = 0.02f;
+    const float nuf = (float)g.nu;
 
     for (int i = 0; i < ZT; ++i)
         accs[i][0][tid] = accs[i][1][tid] = 0.f;
 
     for (int p = 0; p < g.np; ++p) {
-        const float *S = g.src + 3 * p;
-        const float *D = g.det + 3 * p;
-        const float sfx = S[0] / g.sx, sfy = S[1] / g.sy, sfz = S[2] / g.sz;
-        const float ofx = D[0] / g.sx, ofy = D[1] / g.sy, ofz = D[2] / g.sz;
-        const float dd = ofz - sfz;
-        const float ta = dd / (((float)z0 - hz) - sfz);
-        const float tb = dd / (((float)(z0 + nzt) - hz) - sfz);
+        // Per-view terms, uniform over the workgroup and evaluated once per
+        // geometry with the same FP32 operations (scalar loads): source x
+        // over sx, detector corner and far edge x over sx, and the lateral
+        // scales T of the brick's top and bottom z planes (the zs table).
+        const KvConstF *C = kv_const(g.rc + KV_RC * p);
+        const float sfx = C[0], ofx = C[8], efx = C[9];
+        const KvConstF *Zg = kv_const(t.zs + ((size_t)p * nz1 + z0) * KV_ZS);
+        const float ta = Zg[0], tb = Zg[nzt * KV_ZS];
         const float *tx = t.x + (size_t)p * nx1 * g.nu;
         const float *ty = t.y + (size_t)p * ny1 * g.nv;
 
@@ -289,113 +382,36 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             }
             const float u = clampf((sfx + mn - ofx) * scx - 0.5f, -1e6f, 1e6f);
             xbase[tid] = (int)floorf(u) - 2;
-        } else if (tid == TX + 1) {
-            float mn = kInf;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const float c = ((float)(y0 + q * TY) - hy) - sfy;
-                mn = fminf(mn, fminf(c * ta, c * tb));
-            }
-            const float v = clampf((sfy + mn - ofy) * scy - 0.5f, -1e6f, 1e6f);
-            ybase_s = (int)floorf(v) - 2;
         }
         __syncthreads();
-        // Table bands through bounds-checked buffer resources over this
-        // view's tables: entries past a table read 0, and entries whose
-        // detector column or row lies off the detector are never used (the
-        // band slots read below always belong to columns in [0, nu) and rows
-        // in [0, nv)), so no range test is needed.
-        const __amdgpu_buffer_rsrc_t txs =
-            __builtin_amdgcn_make_buffer_rsrc((void *)tx, (short)0, nx1 * g.nu * 4, 0x00020000);
-        const __amdgpu_buffer_rsrc_t tys =
-            __builtin_amdgcn_make_buffer_rsrc((void *)ty, (short)0, ny1 * g.nv * 4, 0x00020000);
-        // a thread's band entries are staged in rounds of SR loads, each
-        // round issued before its LDS writes, so the staging waits for a few
-        // memory latencies instead of one per entry
-        constexpr int NX = (TX + 1) * XBW, NY = (TY + 1) * YBR;
-        const int ybase = ybase_s;
-        auto xload = [&](int e) {
-            const int i = e / XBW, j = e - i * XBW;
-            const unsigned k = (unsigned)(x0 + i), px = (unsigned)(xbase[i] + j);
-            return __uint_as_float(
-                __builtin_amdgcn_raw_buffer_load_b32(txs, (int)((k * (unsigned)g.nu + px) * 4u), 0, 0));
-        };
-        auto yload = [&](int e) {
-            const int c = e / YBR, r = e - c * YBR;
-            const unsigned j = (unsigned)(y0 + c), py = (unsigned)(ybase + r);
-            return __uint_as_float(
-                __builtin_amdgcn_raw_buffer_load_b32(tys, (int)((j * (unsigned)g.nv + py) * 4u), 0, 0));
-        };
-        for (int e0 = tid; e0 < NX; e0 += 256 * SR) {
-            float v[SR];
+        // Table bands through a bounds-checked buffer resource over this
+        // view's x table: entries past the table read 0, and entries whose
+        // detector column lies off the detector are never used (the band
+        // slots read below always belong to columns in [0, nu)), so no range
+        // test is needed.  A thread stages eight consecutive entries of one
+        // band with two 16-byte loads issued together.
+        {
+            const __amdgpu_buffer_rsrc_t txs =
+                __builtin_amdgcn_make_buffer_rsrc((void *)tx, (short)0, nx1 * g.nu * 4, 0x00020000);
+            for (int e = tid; e < (TX + 1) * (XBW / 8); e += 256) {
+                const int i = e / (XBW / 8), j = (e % (XBW / 8)) * 8;
+                const unsigned k = (unsigned)(x0 + i), px = (unsigned)(xbase[i] + j);
+                const int off = (int)((k * (unsigned)g.nu + px) * 4u);
+                const auto v0 = __builtin_amdgcn_raw_buffer_load_b128(txs, off, 0, 0);
+                const auto v1 = __builtin_amdgcn_raw_buffer_load_b128(txs, off + 16, 0, 0);
+                float *d = xb + i * XBS + j;
 #pragma unroll
-            for (int q = 0; q < SR; ++q)
-                v[q] = e0 + 256 * q < NX ? xload(e0 + 256 * q) : 0.f;
-#pragma unroll
-            for (int q = 0; q < SR; ++q) {
-                const int e = e0 + 256 * q;
-                if (e < NX)
-                    xb[(e / XBW) * XBS + e % XBW] = v[q];
-            }
-        }
-        const float *zsp = t.zs + ((size_t)p * nz1 + z0) * KV_ZS;
-        for (int e = tid; e < (nzt + 1) * KV_ZS; e += 256)
-            sm.zsb[e] = zsp[e];
-        for (int e0 = tid; e0 < NY; e0 += 256 * SR) {
-            float v[SR];
-#pragma unroll
-            for (int q = 0; q < SR; ++q)
-                v[q] = e0 + 256 * q < NY ? yload(e0 + 256 * q) : 0.f;
-#pragma unroll
-            for (int q = 0; q < SR; ++q) {
-                const int e = e0 + 256 * q;
-                if (e < NY)
-                    yb[e] = v[q];
+                for (int q = 0; q < 4; ++q) {
+                    d[q] = __uint_as_float(v0[q]);
+                    d[4 + q] = __uint_as_float(v1[q]);
+                }
             }
         }
         __syncthreads();
-
-        const float rdx = (cvx + 0.5f) - sfx, rdy = (cvy + 0.5f) - sfy;
-        const float efx = (D[0] + (float)g.nu * g.pu) / g.sx;
-        const float efy = (D[1] + (float)g.nv * g.pv) / g.sy;
-        // Terms of slice zi that are uniform over the wavefront (its voxels
-        // share y and z), computed once per view by lane zi and read back
-        // with v_readlane: the detector-row range (shadow, intersected with
-        // the +-5 window in mode R, Projector.cu:607-646), whether the voxel
-        // centre's row lies on the widened detector, whether the shadow is
-        // defined, and the z-plane alpha interval.
-        int yr_lo = 0, yr_hi = -1, yflags = 0;
-        float zlz = 0.f, zuz = 0.f, zlzp = 0.f;
-        if (lane < nzt) {
-            const float *Z = sm.zsb + lane * KV_ZS;
-            const float t0 = Z[0], t1 = Z[KV_ZS];
-            const bool hs = t0 > 0.f && t1 > 0.f && t0 < 1e30f && t1 < 1e30f;
-            int f0 = 0, f1 = g.nv - 1;
-            if (hs)
-                shadow_affine(cvy, Z[4], Z[5], Z[KV_ZS + 4], Z[KV_ZS + 5], g.nv, f0, f1);
-            bool yin = true;
-            if (MODE == KV_MODE_R) {
-                const float iy = sfy + rdy * Z[1];
-                yin = iy > ofy - wy && iy < efy + wy;
-                const int pcy = (int)(((iy - ofy) * g.sy) / g.pv);
-                int wly = pcy - half, why = pcy + half;
-                wly = wly < 0 ? 0 : (wly >= g.nv ? g.nv - 1 : wly);
-                why = why < 0 ? 0 : (why >= g.nv ? g.nv - 1 : why);
-                f0 = max(f0, wly);
-                f1 = min(f1, why);
-            }
-            yr_lo = f0;
-            yr_hi = f1;
-            yflags = (yin ? 1 : 0) | (hs ? 2 : 0);
-            const int z = z0 + lane;
-            const float a1z = t.z[p * nz1 + z], a2z = t.z[p * nz1 + z + 1];
-            zlz = fmaxf(0.f, fminf(a1z, a2z));
-            zuz = fminf(1.f, fmaxf(a1z, a2z));
-            zlzp = bmax(0.f, fminf(a1z, a2z));  // lz with a +0 floor in bit order
-        }
         if (!active)
             continue;
 
+        const float rdx = (cvx + 0.5f) - sfx;
         const int xb0 = xbase[lane], xb1 = xbase[lane + 1];
         const float *pr = proj + (size_t)p * g.nu * g.nv;
         const __amdgpu_buffer_rsrc_t prs =
@@ -404,25 +420,57 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
         const float *ly2 = MODE == KV_MODE_A ? t.ly2 + (size_t)p * g.nv : nullptr;
         const float lz2 = MODE == KV_MODE_A ? t.lz2[p] : 0.f;
         const float *xr0 = xb + lane * XBS, *xr1 = xb + (lane + 1) * XBS;
-        const float *yr0 = yb + wv * YBR, *yr1 = yb + (wv + 1) * YBR;
+        // this wavefront's two y-plane rows of the view's y table and its
+        // (y, z)-row terms: uniform addresses in memory this kernel does not
+        // write, read through the constant address space (scalar loads)
+        const KvConstF *ty0 = kv_const(ty + (size_t)y * g.nv), *ty1 = kv_const(ty + (size_t)(y + 1) * g.nv);
+        const KvConstF *YZ = kv_const(t.yz + (((size_t)p * g.nz + z0) * g.ny + y) * KV_YZ);
+        // Detector column of the voxel's lateral edges x = cvx and cvx + 1
+        // on the plane at the top of slice zi (affine map of the per-view
+        // table, bounds only); the bottom plane of slice zi is the top plane
+        // of slice zi + 1, so each plane's pair is evaluated once.
+        float ua0 = __builtin_fmaf(cvx, Zg[2], Zg[3]), ua1 = __builtin_fmaf(cvx + 1.f, Zg[2], Zg[3]);
 
         for (int zi = 0; zi < nzt; ++zi) {
-            const int fl = __builtin_amdgcn_readlane(yflags, zi);
-            if (!(fl & 1))
-                continue;  // voxel centre outside the widened detector rows (whole wavefront)
-            const bool has_shadow = (fl & 2) != 0;
-            const float *Z = sm.zsb + zi * KV_ZS;  // plane z0+zi; Z + KV_ZS is the next plane
-            // shadow of the voxel box on the detector columns (conservative,
-            // see shadow_affine); rows come from the per-view pass
+            const KvConstF *Z = Zg + zi * KV_ZS;  // plane z0+zi; Z + KV_ZS is the next plane
+            const float ub0 = __builtin_fmaf(cvx, Z[KV_ZS + 2], Z[KV_ZS + 3]);
+            const float ub1 = __builtin_fmaf(cvx + 1.f, Z[KV_ZS + 2], Z[KV_ZS + 3]);
+            const float mnx = vmin(ua0, ub0), mxx = vmax(ua1, ub1);
+            ua0 = ub0;
+            ua1 = ub1;
+            // the wavefront's detector rows (empty when mode R's voxel
+            // centre row lies off the widened detector) and z-plane terms
+            const KvConstF *E = YZ + (size_t)zi * g.ny * KV_YZ;
+            const int ly = __float_as_int(E[0]), hyw = __float_as_int(E[1]);
+            if (ly > hyw)
+                continue;
+            const bool has_shadow = __float_as_int(E[2]) != 0;
+            // Shadow of the voxel box on the detector columns: the columns
+            // whose centres lie in [mnx - pad, mxx + pad] (see shadow_affine;
+            // floor(a + 1) for ceil(a) only drops a column lying exactly on
+            // the widened edge, which the padding keeps clear of every ray
+            // that crosses the box).  rlo/rhi before the clamp to the
+            // detector tell whether the shadow lies inside it.
             int fx0 = 0, fx1 = g.nu - 1;
-            if (has_shadow)
-                shadow_affine(cvx, Z[2], Z[3], Z[KV_ZS + 2], Z[KV_ZS + 3], g.nu, fx0, fx1);
+            bool interior = false;
+            if (has_shadow) {
+                const int rlo = cvt_flr(clampf(mnx + (1.f - pad), -1.f, nuf + 2.f));
+                const int rhi = cvt_flr(clampf(mxx + pad, -2.f, nuf + 1.f));
+                fx0 = max(rlo, 0);
+                fx1 = min(rhi, g.nu - 1);
+                interior = rlo >= 0 && rhi <= g.nu - 1;
+            }
             int lx = fx0, hxw = fx1;
-            const int ly = __builtin_amdgcn_readlane(yr_lo, zi), hyw = __builtin_amdgcn_readlane(yr_hi, zi);
-            if (MODE == KV_MODE_R) {
-                // voxel centre on the detector and the +-5 window,
+            if (MODE == KV_MODE_R && !(interior && hxw - lx < 4)) {
+                // Voxel centre on the detector and the +-5 window,
                 // Projector.cu:473-492, 607-646 (az is the reference's
-                // (ofz - sfz) / rdz, from the per-view table)
+                // (ofz - sfz) / rdz, from the per-view table).  Skipped for
+                // a shadow of at most four columns inside the detector: the
+                // voxel centre projects into its shadow (a point of the box
+                // projects inside the box's projection), so the centre lies
+                // within the shadow's width (< 5 pixels) of every shadow
+                // column, the window [pc - 5, pc + 5] contains the shadow and
+                // the centre lies on the detector: both tests pass.
                 const float ix = sfx + rdx * Z[1];
                 if (!(ix > ofx - wx && ix < efx + wx))
                     continue;
@@ -447,11 +495,9 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                     hxw = min(whx, fx1);
                 }
             }
-            if (lx > hxw || ly > hyw)
+            if (lx > hxw)
                 continue;
-            const float lz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlz), zi));
-            const float uz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zuz), zi));
-            const float lzp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlzp), zi));
+            const float lz = E[4], uz = E[5], lzp = E[6];
             // one running sum per voxel across all views, as the reference
             // accumulates (Projector.cu:727-728)
             float a = accs[zi][0][tid], wsm = accs[zi][1][tid];
@@ -460,113 +506,123 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             // unsigned compares (meaningful for the ncol <= 4 it is used with)
             const bool in_band = (unsigned)(lx - xb0) <= (unsigned)(XBW - ncol) &&
                                  (unsigned)(lx - xb1) <= (unsigned)(XBW - ncol);
-            const bool rows_in_band = ly - ybase >= 0 && hyw - ybase < YBR;  // wave-uniform
-            if (ncol <= 4 && in_band && rows_in_band) {
-                // x intervals of the (at most NC) columns stay in registers;
-                // NC is the wavefront's widest column range (2, 3 or 4), so
-                // narrow shadows do not pay for four columns per row
-                auto rows = [&](auto nc_tag) {
-                    constexpr int NC = decltype(nc_tag)::value;
-                    float Lx[NC], Hx[NC], L2[NC];
+            // Columns lx .. lx + 3 all in both bands and on the detector, and
+            // [lx, hxw] the whole shadow (not cut by the window or the
+            // detector edge): the columns after hxw lie outside the widened
+            // shadow, so their weights fail the acceptance test in every row
+            // (w <= 0, exactly the skipped pixels of the shadow argument),
+            // and they serve as the spare columns of a narrower range with no
+            // per-column tests.
+            const bool simple = interior && ncol <= 4 && lx + 3 <= g.nu - 1 &&
+                                (unsigned)(lx - xb0) <= (unsigned)(XBW - 4) &&
+                                (unsigned)(lx - xb1) <= (unsigned)(XBW - 4);
+            // x intervals of the (at most NC) columns stay in registers; NC is
+            // the wavefront's widest column range (2, 3 or 4), so narrow
+            // shadows do not pay for four columns per row
+            auto rows = [&](auto nc_tag, auto simple_tag) {
+                constexpr int NC = decltype(nc_tag)::value;
+                constexpr bool SIMPLE = decltype(simple_tag)::value;
+                float Lx[NC], Hx[NC], L2[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    Lx[c] = 1.f;
+                    Hx[c] = 0.f;  // empty: any weight <= -1
+                    L2[c] = 0.f;
+                    if (SIMPLE || c < ncol) {
+                        const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
+                        Lx[c] = bmax(lzp, bmin(a1x, a2x));
+                        Hx[c] = bmin(uz, bmax(a1x, a2x));
+                        if (MODE == KV_MODE_A && UNIT)
+                            L2[c] = lx2[lx + c];
+                    }
+                }
+                // Projection values through a bounds-checked buffer resource
+                // over this view's plane: the columns of a row are one 32-bit
+                // offset plus immediate offsets (one wide load), columns past
+                // the row end read the next row (never used: their weight is
+                // not positive) and reads past the plane return 0 instead of
+                // faulting.  The next row's values are requested before the
+                // current row is reduced, into two register sets that
+                // alternate between rows (no copies).  (A resource rebuilt
+                // per row on the scalar unit, to save the offset's vector
+                // add, measured 13 % slower: the scalar unit issues at the
+                // vector unit's rate and was the busier of the two.)
+                float va[NC], vb[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    va[c] = vb[c] = 0.f;
+                constexpr bool LOADS = EP != KV_EP_COUNT && !UNIT;
+                int off = (ly * g.nu + lx) * 4 - 4 * g.nu;
+                auto load_row = [&](int py, float (&dst)[NC]) {
+                    (void)py;
+                    off += 4 * g.nu;
+                    if (LOADS) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+                            dst[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
+                    }
+                };
+                auto row = [&](int py, const float (&v)[NC]) {
+                    // the wavefront's y-plane crossings of this row (uniform:
+                    // scalar loads and scalar min/max)
+                    const float a1y = ty0[py], a2y = ty1[py];
+                    // max/min are exact: the reference's grouping
+                    // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
+                    const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
+                    const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
-                        Lx[c] = 1.f;
-                        Hx[c] = 0.f;  // empty: any weight <= -1
-                        L2[c] = 0.f;
-                        if (c < ncol) {
-                            const float a1x = xr0[lx + c - xb0], a2x = xr1[lx + c - xb1];
-                            Lx[c] = bmax(lzp, bmin(a1x, a2x));
-                            Hx[c] = bmin(uz, bmax(a1x, a2x));
-                            if (MODE == KV_MODE_A && UNIT)
-                                L2[c] = lx2[lx + c];
-                        }
-                    }
-                    // Projection values through a bounds-checked buffer
-                    // resource over this view's plane: the columns of a row
-                    // are one 32-bit offset plus immediate offsets, columns
-                    // past the row end read the next row (never used: their
-                    // weight is negative) and reads past the plane return 0
-                    // instead of faulting.  The next row's values are
-                    // requested before the current row is reduced.
-                    // two register sets alternate between rows (no copies)
-                    float va[NC], vb[NC];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                        va[c] = vb[c] = 0.f;
-                    int off = (ly * g.nu + lx) * 4;
-                    constexpr bool LOADS = EP != KV_EP_COUNT && !UNIT;
-                    auto load_row = [&](float (&dst)[NC]) {
-                        if (LOADS) {
-#pragma unroll
-                            for (int c = 0; c < NC; ++c)
-                                dst[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
-                        }
-                    };
-                    auto row = [&](int py, const float (&v)[NC]) {
-                        const int r = py - ybase;
-                        const float a1y = yr0[r], a2y = yr1[r];
-                        // max/min are exact: the reference's grouping
-                        // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
-                        const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
-                        const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            const float w = bmin(Hx[c], hy_) - bmax(Lx[c], ly_);
+                        const float hmin = bmin(Hx[c], hy_), lmax = bmax(Lx[c], ly_);
+                        if (EP == KV_EP_COUNT) {
+                            const float w = hmin - lmax;
+                            wsm = w > tol ? wsm + 1.f : wsm;
+                        } else if (MODE == KV_MODE_R && FW) {
+                            const float wp = __builtin_amdgcn_fmed3f(hmin - lmax, 0.f, 1.f);
+                            wsm = wsm + wp;
+                            a = a + kv_mul_legacy(wp, v[c]);
+                        } else if (MODE == KV_MODE_R) {
+                            const float w = hmin - lmax;
                             const bool ok = w > tol;
-                            // selects, or (FW, mode A) the clamped weight:
-                            // skipped pixels leave the running sums
-                            // bit-identical either way
-                            const float wp = bmax(w, 0.f);
-                            if (EP == KV_EP_COUNT) {
-                                wsm = ok ? wsm + 1.f : wsm;
-                            } else if (MODE == KV_MODE_R && FW) {
-                                wsm = wsm + wp;
-                                a = a + kv_mul_legacy(wp, v[c]);
-                            } else if (MODE == KV_MODE_R) {
-                                wsm = ok ? wsm + w : wsm;
-                                a = ok ? a + w * v[c] : a;
-                            } else if (UNIT) {
-                                const float L = sqrtf((L2[c] + lyy) + lz2);
-                                a = a + wp * L;
-                            } else {
-                                // input already multiplied by the ray's
-                                // length factor (kv_launch_scale_a)
-                                a = a + kv_mul_legacy(wp, v[c]);
-                            }
+                            wsm = ok ? wsm + w : wsm;
+                            a = ok ? a + w * v[c] : a;
+                        } else if (UNIT) {
+                            const float wp = __builtin_amdgcn_fmed3f(hmin - lmax, 0.f, 1.f);
+                            const float L = sqrtf((L2[c] + lyy) + lz2);
+                            a = a + wp * L;
+                        } else {
+                            // input already multiplied by the ray's length
+                            // factor (kv_launch_scale_a)
+                            const float wp = __builtin_amdgcn_fmed3f(hmin - lmax, 0.f, 1.f);
+                            a = a + kv_mul_legacy(wp, v[c]);
                         }
-                    };
-                    // each row's values are requested before the previous
-                    // row is reduced
-                    load_row(va);
-                    int py = ly;
-                    for (; py < hyw; py += 2) {
-                        off += 4 * g.nu;
-                        load_row(vb);
-                        row(py, va);
-                        off += 4 * g.nu;
-                        load_row(va);
-                        row(py + 1, vb);
                     }
-                    if (py == hyw)
-                        row(py, va);
                 };
+                load_row(ly, va);
+                int py = ly;
+                for (; py < hyw; py += 2) {
+                    load_row(py + 1, vb);
+                    row(py, va);
+                    load_row(py + 2, va);
+                    row(py + 1, vb);
+                }
+                if (py == hyw)
+                    row(py, va);
+            };
+            auto rows_nc = [&](auto simple_tag) {
                 if (__all(ncol <= 2))
-                    rows(std::integral_constant<int, 2>{});
+                    rows(std::integral_constant<int, 2>{}, simple_tag);
                 else if (__all(ncol <= 3))
-                    rows(std::integral_constant<int, 3>{});
+                    rows(std::integral_constant<int, 3>{}, simple_tag);
                 else
-                    rows(std::integral_constant<int, 4>{});
+                    rows(std::integral_constant<int, 4>{}, simple_tag);
+            };
+            if (__all(simple)) {
+                rows_nc(std::true_type{});
+            } else if (ncol <= 4 && in_band) {
+                rows_nc(std::false_type{});
             } else {
                 for (int py = ly; py <= hyw; ++py) {
-                    const int r = py - ybase;
-                    float a1y, a2y;
-                    if (r >= 0 && r < YBR) {
-                        a1y = yr0[r];
-                        a2y = yr1[r];
-                    } else {
-                        a1y = ty[(size_t)y * g.nv + py];
-                        a2y = ty[(size_t)(y + 1) * g.nv + py];
-                    }
+                    const float a1y = ty0[py], a2y = ty1[py];
                     const float lo = fmaxf(lz, fminf(a1y, a2y));
                     const float up = fminf(uz, fmaxf(a1y, a2y));
                     const float *prow = pr + (size_t)py * g.nu;
@@ -740,6 +796,7 @@ hipError_t kv_launch_tables_r(const KvGeom &g, const KvTables &t, hipStream_t s)
     tables_r_kernel<<<2048, 256, 0, s>>>(g, t);
     const int n = g.np * (g.nz + 1);
     zs_kernel<<<(n + 255) / 256, 256, 0, s>>>(g, t.zs);
+    yz_kernel<KV_MODE_R><<<1024, 256, 0, s>>>(g, t);
     return hipGetLastError();
 }
 
@@ -748,6 +805,7 @@ hipError_t kv_launch_tables_a(const KvGeom &g, const KvTables &t, hipStream_t s)
     tables_a_kernel<<<2048, 256, 0, s>>>(g, t);
     const int n = g.np * (g.nz + 1);
     zs_kernel<<<(n + 255) / 256, 256, 0, s>>>(g, t.zs);
+    yz_kernel<KV_MODE_A><<<1024, 256, 0, s>>>(g, t);
     return hipGetLastError();
 }
 

@@ -23,24 +23,27 @@ struct KvGeom {
     // kernels would: rc[KV_RC*p + 0..6] = source x, y, z over the spacing,
     // dz (detector z / sz - source z / sz), the z-face alphas
     // (-nz/2 - sz_p) / dz and (nz/2 - sz_p) / dz and 1 / dz (the last three
-    // 0 when |dz| <= 1e-6); then qx[p][u] = (det x + (u + 0.5) pu) / sx at
-    // rc + KV_RC*np and qy[p][v] after it.
+    // 0 when |dz| <= 1e-6); for the backprojectors rc[KV_RC*p + 8, 9] =
+    // detector corner x over sx and the far detector edge
+    // (det x + nu * pu) / sx (Projector.cu:473-492); then
+    // qx[p][u] = (det x + (u + 0.5) pu) / sx at rc + KV_RC*np and qy[p][v]
+    // after it.
     const float *rc;
     int walk_fast;  // forward walk may use end thresholds and 24-bit indices (see kv_fp.hip)
     int bp_posz;    // every view's z planes at alpha >= 1e-6: clamped backprojection weights (see kv_bp.hip)
 };
 
-constexpr int KV_RC = 8;
+constexpr int KV_RC = 16;
 
 // Per-view plane-crossing tables.  For mode R an entry is the reference's
 // alpha1/alpha2 of Projector.cu:653-669 for plane k of a pixel's ray:
-//   x[p][u][k] = (|rx| > 1e-24) ? ((k - nx/2) - sx) / rx : 0
+//   x[p][k][u] = (|rx| > 1e-24) ? ((k - nx/2) - sx) / rx : 0
 // For mode A it is the mode-A plane crossing ((k - n/2) - s) * (1/d), with
 // +-INF encoding of rays parallel to the planes, plus the squared mm
 // components of each ray (lx2[p][u], ly2[p][v], lz2[p]).
 struct KvTables {
-    float *x = nullptr;   // [np][nu][nx+1]
-    float *y = nullptr;   // [np][nv][ny+1]
+    float *x = nullptr;   // [np][nx+1][nu]
+    float *y = nullptr;   // [np][ny+1][nv]
     float *z = nullptr;   // [np][nz+1]
     float *lx2 = nullptr; // mode A: [np][nu]
     float *ly2 = nullptr; // mode A: [np][nv]
@@ -53,8 +56,19 @@ struct KvTables {
     //          coordinate c on plane l, u ~= Kx0 + c * Kx1 (bounds only)
     //  [4] Ky1, [5] Ky0: the same for detector rows
     float *zs = nullptr;
+    // Per view, z slice l and y row j (index (p * nz + l) * ny + j), KV_YZ
+    // words, the backprojectors' terms shared by the voxels of one (y, z)
+    // row: [0] first and [1] last detector row of the voxel box's shadow
+    // (intersected with the +-5 window around the voxel centre's row in mode
+    // R; empty, first > last, when mode R's centre row lies off the widened
+    // detector), [2] 1 if the shadow is defined (both planes in front of the
+    // source), as ints; [4] lz = max(0, min z-plane alpha), [5] uz =
+    // min(1, max z-plane alpha), [6] lz with a +0 floor in bit order, as
+    // floats.
+    float *yz = nullptr;
 };
 constexpr int KV_ZS = 8;
+constexpr int KV_YZ = 8;
 
 // Epilogues of the projection kernels: plain result, fused MLEM ratio/update,
 // fused SART residual/update, or the count of accepted (ray, voxel) pairs.
