@@ -131,9 +131,20 @@ bool bp_posz_ok(const kv_geometry &q)
 // z terms of :54-61 and :186), so every value is bit-identical.
 std::vector<float> ray_constants(const kv_geometry &q)
 {
-    std::vector<float> rc((size_t)q.np * (KV_RC + q.nu + q.nv), 0.f);
-    float *qx = rc.data() + (size_t)q.np * KV_RC, *qy = qx + (size_t)q.np * q.nu;
-    const float hz = (float)q.nz / 2.f;
+    std::vector<float> rc((size_t)q.np * (KV_RC + 4 * q.nu + 4 * q.nv), 0.f);
+    float *qx = rc.data() + (size_t)q.np * KV_RC, *qy = qx + (size_t)q.np * q.nu * 4;
+    const float hx = (float)q.nx / 2.f, hy = (float)q.ny / 2.f, hz = (float)q.nz / 2.f;
+    // per detector column (row): centre coordinate, the reference's face
+    // alphas of the axis (Projector.cu:54-61, 0 unless |d| > 1e-6) and 1/d
+    // (0 when d = 0; the mode-R walk uses it only when |d| > 1e-6)
+    auto axis = [](float *o, float q, float s, float h) {
+        const float d = q - s;
+        const bool ok = std::fabs(d) > 1e-6f;
+        o[0] = q;
+        o[1] = ok ? (-h - s) / d : 0.f;
+        o[2] = ok ? (h - s) / d : 0.f;
+        o[3] = d != 0.f ? 1.f / d : 0.f;
+    };
     for (int p = 0; p < q.np; ++p) {
         const float *S = q.src + 3 * p, *D = q.det + 3 * p;
         const float sz = S[2] / q.sz, dz = D[2] / q.sz - sz;
@@ -149,9 +160,9 @@ std::vector<float> ray_constants(const kv_geometry &q)
         c[8] = D[0] / q.sx;
         c[9] = (D[0] + (float)q.nu * q.pu) / q.sx;
         for (int u = 0; u < q.nu; ++u)
-            qx[(size_t)p * q.nu + u] = (D[0] + ((float)u + 0.5f) * q.pu) / q.sx;
+            axis(qx + ((size_t)p * q.nu + u) * 4, (D[0] + ((float)u + 0.5f) * q.pu) / q.sx, c[0], hx);
         for (int v = 0; v < q.nv; ++v)
-            qy[(size_t)p * q.nv + v] = (D[1] + ((float)v + 0.5f) * q.pv) / q.sy;
+            axis(qy + ((size_t)p * q.nv + v) * 4, (D[1] + ((float)v + 0.5f) * q.pv) / q.sy, c[1], hy);
     }
     return rc;
 }
@@ -362,7 +373,7 @@ kv_status kv_set_geometry(kv_context *ctx, const kv_geometry *geom)
     kv_status st;
     if ((st = dev_alloc((void **)&ctx->d_det, (size_t)q.np * 12)) ||
         (st = dev_alloc((void **)&ctx->d_src, (size_t)q.np * 12)) ||
-        (st = dev_alloc((void **)&ctx->d_rc, (size_t)q.np * (KV_RC + q.nu + q.nv) * 4)) ||
+        (st = dev_alloc((void **)&ctx->d_rc, (size_t)q.np * (KV_RC + 4 * q.nu + 4 * q.nv) * 4)) ||
         (st = dev_alloc((void **)&ctx->d_ratio, (size_t)nray * 4))) {
         release_geometry(ctx);
         return st;

@@ -27,6 +27,13 @@
 
 #include "kv_internal.h"
 
+// The structured-buffer load intrinsic (no clang builtin spells it):
+// rsrc, record index, byte offset within the record, scalar offset, cache
+// policy.
+typedef int KvRsrc __attribute__((ext_vector_type(4)));
+__device__ float kv_struct_load_f32(KvRsrc rsrc, int vindex, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.struct.buffer.load.f32");
+
 namespace {
 
 constexpr float kInf = __builtin_inff();
@@ -44,22 +51,32 @@ struct RayF {
     float sx, sy, sz;    // source, floating-voxel units
     float dx, dy, dz;    // source -> pixel centre
     float hx, hy, hz;    // half sizes
+    float a1x, aNx, ivx; // x-face alphas (0 when |dx| <= 1e-6) and 1/dx (0 when dx = 0)
+    float a1y, aNy, ivy; // the same for y
     float a1z, aNz, stz; // z-face alphas and 1/dz (0 when |dz| <= 1e-6)
 };
 
-// Projector.cu:303-323.  The terms that depend on the view alone, and the
-// pixel-centre coordinates, come from the geometry's ray constants (the same
-// FP32 operations evaluated once on the host).
+// Projector.cu:303-323.  The terms that depend on the view alone, and those
+// of the pixel's detector column and row (its centre coordinate, the face
+// alphas and the reciprocal of the direction along that axis), come from the
+// geometry's ray constants: the same FP32 operations evaluated once on the
+// host.
 __device__ __forceinline__ void ray_setup(const KvGeom &g, int p, int u, int v, RayF &r)
 {
     const float *C = g.rc + KV_RC * p;
     r.sx = C[0];
     r.sy = C[1];
     r.sz = C[2];
-    const float qx = g.rc[(size_t)g.np * KV_RC + (size_t)p * g.nu + u];
-    const float qy = g.rc[(size_t)g.np * (KV_RC + g.nu) + (size_t)p * g.nv + v];
-    r.dx = qx - r.sx;
-    r.dy = qy - r.sy;
+    const float4 cx = *(const float4 *)(g.rc + (size_t)g.np * KV_RC + ((size_t)p * g.nu + u) * 4);
+    const float4 cy = *(const float4 *)(g.rc + (size_t)g.np * (KV_RC + 4 * g.nu) + ((size_t)p * g.nv + v) * 4);
+    r.dx = cx.x - r.sx;
+    r.dy = cy.x - r.sy;
+    r.a1x = cx.y;
+    r.aNx = cx.z;
+    r.ivx = cx.w;
+    r.a1y = cy.y;
+    r.aNy = cy.z;
+    r.ivy = cy.w;
     r.dz = C[3];
     r.hx = (float)g.nx / 2.f;
     r.hy = (float)g.ny / 2.f;
@@ -84,6 +101,15 @@ __device__ __forceinline__ void axis_range(float d, float s, float h, int n, flo
     hi = mx;
 }
 
+// a * b + c on the low 24 bits of a and b, one instruction (the compiler
+// would otherwise form two products and a three-way add).
+__device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
+{
+    unsigned r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Minimum of three run heads as one v_min3_f32.  The heads are finite or
 // +INF (never NaN), so the IEEE-mode operand canonicalisation the compiler
 // would add for fminf is not needed.
@@ -92,6 +118,20 @@ __device__ __forceinline__ float min3f(float a, float b, float c)
     float m;
     asm("v_min3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(a), "v"(b), "v"(c));
     return m;
+}
+
+// Volume gathers by voxel index: a structured buffer of 4-byte records,
+// bounds-checked by index (an index at or past the voxel count reads 0), so
+// the gather takes the linear index with no byte-offset shift.
+__device__ __forceinline__ KvRsrc voxel_rsrc(const KvGeom &g, const float *vol)
+{
+    const unsigned long long a = (unsigned long long)vol;
+    KvRsrc r;
+    r.x = (int)(unsigned)a;
+    r.y = (int)(((unsigned)(a >> 32) & 0xFFFFu) | (4u << 16));  // record stride 4 bytes
+    r.z = g.nx * g.ny * g.nz;                                   // records (at most 2^30)
+    r.w = 0x00020000;
+    return r;
 }
 
 // Raw buffer resource over the volume (byte range clamped to 32 bits; the
@@ -121,12 +161,9 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
 {
     const float tol = 1e-6f;
     const bool okx = fabsf(r.dx) > tol, oky = fabsf(r.dy) > tol, okz = fabsf(r.dz) > tol;
-    const float a1x = okx ? (-r.hx - r.sx) / r.dx : 0.f;
-    const float a1y = oky ? (-r.hy - r.sy) / r.dy : 0.f;
-    const float a1z = r.a1z;  // okz ? (-hz - sz) / dz : 0
-    const float aNx = okx ? (r.hx - r.sx) / r.dx : 0.f;
-    const float aNy = oky ? (r.hy - r.sy) / r.dy : 0.f;
-    const float aNz = r.aNz;
+    // okx ? (-hx - sx) / dx : 0 and the like, from the ray constants
+    const float a1x = r.a1x, a1y = r.a1y, a1z = r.a1z;
+    const float aNx = r.aNx, aNy = r.aNy, aNz = r.aNz;
     const float amin = fmaxf(fmaxf(0.f, fminf(a1x, aNx)), fmaxf(fminf(a1y, aNy), fminf(a1z, aNz)));
     const float amax = fminf(fminf(1.f, fmaxf(a1x, aNx)), fminf(fmaxf(a1y, aNy), fmaxf(a1z, aNz)));
     if (fsupeq(amin, amax, 1e-7f))
@@ -143,13 +180,13 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     int cx = 0, cy = 0, cz = 0;
     float ax = 0.f, ay = 0.f, az = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
     if (okx) {
-        const float st = 1.f / r.dx;
+        const float st = r.ivx;  // 1 / dx
         ax = st * (-r.hx - r.sx + (float)(r.dx > 0 ? lox : hix));
         ix = fabsf(st);
         cx = hix - lox + 1;
     }
     if (oky) {
-        const float st = 1.f / r.dy;
+        const float st = r.ivy;  // 1 / dy
         ay = st * (-r.hy - r.sy + (float)(r.dy > 0 ? loy : hiy));
         iy = fabsf(st);
         cy = hiy - loy + 1;
@@ -186,19 +223,24 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     // segments are unchanged and one run's increment and end threshold are
     // selected instead of stepping all three.
     auto advance1 = [&](float m) {
-        const bool px = ax == m;
-        const bool py = !px && ay == m;
-        const bool pz = !px && !py;
+        // Two vector compares of bit patterns (the scalar unit combines the
+        // masks; a float compare's negation would be a third compare, as it
+        // must be true for NaN).  m is one of the three heads, so its bits
+        // equal those of the first run that holds it; heads that equal m as
+        // floats with other bits (+0 and -0) only order the consumption of
+        // equal list entries, which bound zero-width segments.
+        const bool px = __float_as_uint(ax) == __float_as_uint(m);
+        const bool qy = __float_as_uint(ay) == __float_as_uint(m);
         // selects of values (a conditional of the captured lvalues would
         // select their addresses and load through them)
         const float vix = ix, viy = iy, viz = iz, vtx = tx, vty = ty, vtz = tz;
-        const float inc = px ? vix : (py ? viy : viz);
-        const float thr = px ? vtx : (py ? vty : vtz);
+        const float inc = px ? vix : (qy ? viy : viz);
+        const float thr = px ? vtx : (qy ? vty : vtz);
         float nv = m + inc;
         nv = nv > thr ? kInf : nv;
         ax = px ? nv : ax;
-        ay = py ? nv : ay;
-        az = pz ? nv : az;
+        ay = (!px && qy) ? nv : ay;
+        az = (!px && !qy) ? nv : az;
     };
     // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
     auto advance = [&](float m) {
@@ -236,7 +278,7 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         const int vy = (int)__builtin_fma(sum, hdy, by);
         const int vz = (int)__builtin_fma(sum, hdz, bz);
         const bool ok = (unsigned)vx < nx && (unsigned)vy < ny && (unsigned)vz < nz && w > tol;
-        const unsigned k = FAST ? __umul24((unsigned)vz, nxy) + __umul24((unsigned)vy, nx) + (unsigned)vx
+        const unsigned k = FAST ? mad_u24((unsigned)vz, nxy, mad_u24((unsigned)vy, nx, (unsigned)vx))
                                 : ((unsigned)vz * ny + (unsigned)vy) * nx + (unsigned)vx;
         visit(ok, k, w);
     };
@@ -348,10 +390,10 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
     // and adds +0 to tw and kv_mul_legacy(+0, value) = +-0 to tot (for any
     // value, INF and NaN included); neither sum is ever -0 (they start at
     // +0 and x + (-x) rounds to +0), so both are left bit-identical.
-    const __amdgpu_buffer_rsrc_t vrs = volume_rsrc(g, vol);
+    const KvRsrc vrs = voxel_rsrc(g, vol);
     float pw = 0.f, pv = 0.f;
     auto acc = [&](bool ok, unsigned k, float w) {
-        const float val = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vrs, (int)(k << 2), 0, 0));
+        const float val = kv_struct_load_f32(vrs, (int)k, 0, 0, 0);
         tw = tw + pw;
         tot = tot + kv_mul_legacy(pw, pv);
         pw = ok ? w : 0.f;
@@ -498,9 +540,11 @@ __device__ __forceinline__ void ray_setup_a(const KvGeom &g, int p, int u, int v
     r.d[0] = f.dx; r.d[1] = f.dy; r.d[2] = f.dz;
     r.h[0] = f.hx; r.h[1] = f.hy; r.h[2] = f.hz;
     r.n[0] = g.nx; r.n[1] = g.ny; r.n[2] = g.nz;
+    r.inv[0] = f.ivx;
+    r.inv[1] = f.ivy;
+    r.inv[2] = r.d[2] != 0.f ? 1.f / r.d[2] : 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        r.inv[c] = r.d[c] != 0.f ? 1.f / r.d[c] : 0.f;
         r.fixed[c] = 0;
         if (r.d[c] == 0.f) {
             const float pos = r.s[c] + r.h[c];

@@ -25,9 +25,12 @@ struct KvGeom {
     // (-nz/2 - sz_p) / dz and (nz/2 - sz_p) / dz and 1 / dz (the last three
     // 0 when |dz| <= 1e-6); for the backprojectors rc[KV_RC*p + 8, 9] =
     // detector corner x over sx and the far detector edge
-    // (det x + nu * pu) / sx (Projector.cu:473-492); then
-    // qx[p][u] = (det x + (u + 0.5) pu) / sx at rc + KV_RC*np and qy[p][v]
-    // after it.
+    // (det x + nu * pu) / sx (Projector.cu:473-492); then per view and
+    // detector column, at rc + KV_RC*np, four floats: the pixel-centre
+    // coordinate qx = (det x + (u + 0.5) pu) / sx, the x-face alphas
+    // (-nx/2 - sx_p) / dx and (nx/2 - sx_p) / dx (0 unless |dx| > 1e-6,
+    // dx = qx - sx_p) and 1 / dx (0 when dx = 0); then the same per view
+    // and detector row for y.
     const float *rc;
     int walk_fast;  // forward walk may use end thresholds and 24-bit indices (see kv_fp.hip)
     int bp_posz;    // every view's z planes at alpha >= 1e-6: clamped backprojection weights (see kv_bp.hip)
