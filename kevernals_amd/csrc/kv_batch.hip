@@ -59,6 +59,10 @@ void layout_items(kv_batch *b)
         P.fp_chunk = span / KV_XCDS;
         P.fp_tx = ((P.g.nu + wu - 1) / wu + span - 1) / span * span;
         P.fp_ty = (P.g.nv + 4 * L - 1) / (4 * L);
+        P.fp_nch = P.fp_tx / span;
+        P.fp_dchunk = kv_udiv((unsigned)P.fp_chunk);
+        P.fp_dnch = kv_udiv((unsigned)P.fp_nch);
+        P.fp_dty = kv_udiv((unsigned)P.fp_ty);
         P.fp_first = fp;
         fp += (long long)P.g.np * P.fp_ty * (P.fp_tx / KV_XCDS);
         P.bp_bx = (P.g.nx + 63) / 64;

@@ -153,7 +153,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const KvGeom &g, c
 // (n + 2) * (max|source| + n/2 + 2) < 2^22 in voxel units.  A run's values are
 // a0 + j*inc accumulated in FP32 with |a| <= (|s| + n/2) * inc, so after j <= n+1
 // additions the accumulated rounding error stays below inc/4: the value
-// T = a0 + (c - 1/2) * inc (FP64, rounded once to FP32) then lies strictly
+// T = a0 + (c - 1/2) * inc (rounded once to FP32) then lies strictly
 // between the run's last value and the one after it, and "exhausted" is one
 // comparison with T instead of a count.  Without FAST each run counts down.
 template <bool FAST, class Visit>
@@ -201,9 +201,13 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
         cz = dzr >= 2 ? dzr - 1 : 1;
     }
     const int cf = n - cx - cy - cz;  // 1.0 fillers left unsorted-over
-    const float tx = FAST ? (float)((double)ax + ((double)cx - 0.5) * (double)ix) : 0.f;
-    const float ty = FAST ? (float)((double)ay + ((double)cy - 0.5) * (double)iy) : 0.f;
-    const float tz = FAST ? (float)((double)az + ((double)cz - 0.5) * (double)iz) : 0.f;
+    // T = a0 + (c - 1/2) * inc rounded once to FP32 (one fma: c - 1/2 is
+    // exact): |T| < 2^22 / (n + 2) * inc under FAST, so the rounding moves T
+    // by less than inc / (4 (n + 2)), far inside the inc / 4 margins above
+    // (any T in the band separates the same run values).
+    const float tx = FAST ? __builtin_fmaf((float)cx - 0.5f, ix, ax) : 0.f;
+    const float ty = FAST ? __builtin_fmaf((float)cy - 0.5f, iy, ay) : 0.f;
+    const float tz = FAST ? __builtin_fmaf((float)cz - 0.5f, iz, az) : 0.f;
 
     // A run's head holds +INF once the run is exhausted, so the next list
     // entry is the plain minimum of the heads.  All heads equal to the
@@ -421,14 +425,19 @@ __device__ __forceinline__ void fp_r_ray(const KvGeom &g, const float *__restric
 // Pixel of lane `tid` in forward tile (tx, ty) for tile layout L (1, 2, 4 or
 // 8): each wavefront covers 64/L consecutive detector columns of L rows, the
 // four wavefronts of a workgroup are stacked along v; a tile is (64/L) x 4L
-// pixels.
+// pixels.  64/L is a power of two: the lane splits by mask and shift.
 __device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int tid, int &u, int &v)
 {
     const int lane = tid & 63, wv = tid >> 6;
-    const int wu = 64 / layout;
-    u = tx * wu + (lane % wu);
-    v = ty * (4 * layout) + wv * layout + lane / wu;
+    const int wu = 64 / layout;  // uniform
+    const int lw = __builtin_ctz((unsigned)wu);
+    u = tx * wu + (lane & (wu - 1));
+    v = ty * (4 * layout) + wv * layout + (lane >> lw);
 }
+
+// n / d for n < 2^31 with the divisor's precomputed multiplier (KvUDiv); on
+// wavefront-uniform operands it runs on the scalar unit.
+__device__ __forceinline__ unsigned udiv(unsigned n, KvUDiv d) { return (__umulhi(n, d.m) + n) >> d.l; }
 
 // Forward tile of this workgroup in an XCD-aware order.  Workgroups are
 // handed to the eight XCDs round-robin by linear id.  The launch is
@@ -446,25 +455,30 @@ __device__ __forceinline__ void fp_tile_pixel(int layout, int tx, int ty, int ti
 // detector's columns hold ray lengths in a V profile cut off at the edges
 // of the volume's shadow.  gxp = tile columns rounded up to a multiple of
 // 8 x chunk; tiles of padding columns (u past the detector) exit at the
-// kernel's bounds test.  A launch covers views p0, p0 + 1, ...
-__device__ __forceinline__ void fp_xcd_tile(int gxp, int gy, int chunk, int p0, int &tx, int &ty, int &p)
+// kernel's bounds test.  A launch covers views p0, p0 + 1, ...  The
+// divisions by chunk, the stripe count and gy use precomputed multipliers
+// (they were a tenth of the vector instructions a wavefront issues before
+// its walk).
+__device__ __forceinline__ void fp_xcd_tile(const KvFpMap &fm, int &tx, int &ty, int &p)
 {
     constexpr unsigned X = KV_XCDS;
-    const unsigned w = blockIdx.x, c = (unsigned)chunk, nch = (unsigned)gxp / (X * c);
-    const unsigned j = w / X, t = j % c, jc = j / c, ci = jc % nch, r = jc / nch;
-    tx = (int)((ci * X + (w + r) % X) * c + t);
-    ty = (int)(r % (unsigned)gy);
-    p = p0 + (int)(r / (unsigned)gy);
+    const unsigned w = blockIdx.x;
+    const unsigned j = w / X;
+    const unsigned jc = udiv(j, fm.dchunk), t = j - jc * (unsigned)fm.chunk;
+    const unsigned r = udiv(jc, fm.dnch), ci = jc - r * (unsigned)fm.nch;
+    tx = (int)((ci * X + (w + r) % X) * (unsigned)fm.chunk + t);
+    const unsigned q = udiv(r, fm.dgy);
+    ty = (int)(r - q * (unsigned)fm.gy);
+    p = fm.p0 + (int)q;
 }
 
 template <int EP>
 __global__ __launch_bounds__(256) void fp_r_kernel(KvGeom g, const float *__restrict__ vol, float *__restrict__ out,
-                                                   const float *__restrict__ data, float relax, int layout,
-                                                   int gxp, int gy, int chunk, int p0)
+                                                   const float *__restrict__ data, float relax, KvFpMap fm)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
-    fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
+    fp_xcd_tile(fm, tx, ty, p);
+    fp_tile_pixel(fm.layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
     fp_r_ray<EP>(g, vol, out, data, relax, p, u, v);
@@ -501,11 +515,15 @@ __device__ __forceinline__ void fp_batch_tiles(const KvProblem *__restrict__ pro
             const KvProblem &P = probs[pi];
             if (iteration >= P.iters)
                 continue;
-            const long long local = item - P.fp_first;
-            const long long ch = P.fp_chunk, nch = P.fp_tx / (KV_XCDS * P.fp_chunk);
-            const long long jc = local / ch, r = jc / nch;
-            const int tx = (int)(((jc - r * nch) * KV_XCDS + k) * ch + (local - jc * ch));
-            const int ty = (int)(r % P.fp_ty), p = (int)(r / P.fp_ty);
+            // item indices within a reconstruction stay far below 2^31 (it
+            // has fewer than 2^31 rays, 256 per tile): 32-bit divisions by
+            // the precomputed multipliers
+            const unsigned local = (unsigned)(item - P.fp_first);
+            const unsigned ch = (unsigned)P.fp_chunk, nch = (unsigned)P.fp_nch;
+            const unsigned jc = udiv(local, P.fp_dchunk), r = udiv(jc, P.fp_dnch);
+            const int tx = (int)(((jc - r * nch) * KV_XCDS + (unsigned)k) * ch + (local - jc * ch));
+            const unsigned q = udiv(r, P.fp_dty);
+            const int ty = (int)(r - q * (unsigned)P.fp_ty), p = (int)q;
             int u, v;
             fp_tile_pixel(P.fp_layout, tx, ty, threadIdx.x, u, v);
             if (u < P.g.nu && v < P.g.nv)
@@ -725,12 +743,11 @@ __device__ __forceinline__ void fp_a_ray(const KvGeom &g, const KvTables &t, con
 
 template <int EP>
 __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, KvTables t, const float *__restrict__ vol,
-                                                   float *__restrict__ out, const float *__restrict__ data, int layout,
-                                                   int gxp, int gy, int chunk, int p0)
+                                                   float *__restrict__ out, const float *__restrict__ data, KvFpMap fm)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
-    fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
+    fp_xcd_tile(fm, tx, ty, p);
+    fp_tile_pixel(fm.layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
     fp_a_ray<EP>(g, t, vol, out, data, p, u, v);
@@ -745,11 +762,11 @@ struct KvMulti {
 };
 
 template <int NV>
-__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, int layout, int gxp, int gy, int chunk, int p0)
+__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, KvFpMap fm)
 {
     int tx, ty, p, u, v;
-    fp_xcd_tile(gxp, gy, chunk, p0, tx, ty, p);
-    fp_tile_pixel(layout, tx, ty, threadIdx.x, u, v);
+    fp_xcd_tile(fm, tx, ty, p);
+    fp_tile_pixel(fm.layout, tx, ty, threadIdx.x, u, v);
     if (u >= g.nu || v >= g.nv)
         return;
     RayA r;
@@ -818,6 +835,7 @@ __global__ void trace_kernel(KvGeom g, int mode, int nrays, const int32_t *__res
 // launch within 2^32 threads.
 struct FpGrid {
     int gxp, gy, chunk, views;
+    KvFpMap fm{};
     FpGrid(const KvGeom &g, int layout)
     {
         const int wu = 64 / layout, wv = 4 * layout;
@@ -826,6 +844,19 @@ struct FpGrid {
         gy = (g.nv + wv - 1) / wv;
         const unsigned long long per_view = (unsigned long long)gxp * (unsigned long long)gy * 256ull;
         views = (int)std::min<unsigned long long>((unsigned long long)g.np, 0xFFFFFFFFull / per_view);
+        fm.layout = layout;
+        fm.chunk = chunk;
+        fm.nch = gxp / (KV_XCDS * chunk);
+        fm.gy = gy;
+        fm.dchunk = kv_udiv(chunk);
+        fm.dnch = kv_udiv(fm.nch);
+        fm.dgy = kv_udiv(gy);
+    }
+    KvFpMap map(int p0) const
+    {
+        KvFpMap m = fm;
+        m.p0 = p0;
+        return m;
     }
     // Runs launch(blocks, p0) for each range of views; a view too large for
     // one launch is an invalid configuration.
@@ -853,16 +884,16 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
     return fg.run(g, [&](unsigned nb, int p0) {
         switch (epilogue) {
         case KV_EP_PLAIN:
-            fp_r_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_r_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, vol, out, data, relax, fg.map(p0));
             break;
         case KV_EP_MLEM:
-            fp_r_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_r_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, vol, out, data, relax, fg.map(p0));
             break;
         case KV_EP_COUNT:
-            fp_r_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_r_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, vol, out, data, relax, fg.map(p0));
             break;
         default:
-            fp_r_kernel<KV_EP_SART><<<nb, 256, 0, s>>>(g, vol, out, data, relax, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_r_kernel<KV_EP_SART><<<nb, 256, 0, s>>>(g, vol, out, data, relax, fg.map(p0));
             break;
         }
     });
@@ -877,9 +908,9 @@ hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols
     const FpGrid fg(g, L);
     return fg.run(g, [&](unsigned nb, int p0) {
         switch (n) {
-        case 1: fp_a_multi_kernel<1><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
-        case 2: fp_a_multi_kernel<2><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
-        default: fp_a_multi_kernel<3><<<nb, 256, 0, s>>>(g, m, L, fg.gxp, fg.gy, fg.chunk, p0); break;
+        case 1: fp_a_multi_kernel<1><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
+        case 2: fp_a_multi_kernel<2><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
+        default: fp_a_multi_kernel<3><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
         }
     });
 }
@@ -891,11 +922,11 @@ hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, 
     const FpGrid fg(g, L);
     return fg.run(g, [&](unsigned nb, int p0) {
         if (epilogue == KV_EP_PLAIN)
-            fp_a_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_a_kernel<KV_EP_PLAIN><<<nb, 256, 0, s>>>(g, t, vol, out, data, fg.map(p0));
         else if (epilogue == KV_EP_COUNT)
-            fp_a_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_a_kernel<KV_EP_COUNT><<<nb, 256, 0, s>>>(g, t, vol, out, data, fg.map(p0));
         else
-            fp_a_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, t, vol, out, data, L, fg.gxp, fg.gy, fg.chunk, p0);
+            fp_a_kernel<KV_EP_MLEM><<<nb, 256, 0, s>>>(g, t, vol, out, data, fg.map(p0));
     });
 }
 

@@ -38,6 +38,31 @@ struct KvGeom {
 
 constexpr int KV_RC = 16;
 
+// Unsigned division by a divisor fixed for a launch: n / d =
+// (umulhi(n, m) + n) >> l for every n < 2^31, with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1 (the round-up method of Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994;
+// the 33-bit sum cannot overflow for n < 2^31).
+struct KvUDiv {
+    unsigned m, l;
+};
+inline KvUDiv kv_udiv(unsigned d)
+{
+    unsigned l = 0;
+    while (l < 31 && (1ull << l) < d)
+        ++l;
+    const unsigned long long m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return KvUDiv{(unsigned)m, l};
+}
+
+// Mapping of a one-dimensional forward launch onto tiles (kv_fp.hip
+// fp_xcd_tile): tile layout, stripe width in tiles, stripes per tile row and
+// tile rows per view with their divisors, first view of the launch.
+struct KvFpMap {
+    int layout, chunk, nch, gy, p0;
+    KvUDiv dchunk, dnch, dgy;
+};
+
 // Per-view plane-crossing tables.  For mode R an entry is the reference's
 // alpha1/alpha2 of Projector.cu:653-669 for plane k of a pixel's ray:
 //   x[p][k][u] = (|rx| > 1e-24) ? ((k - nx/2) - sx) / rx : 0
@@ -95,6 +120,8 @@ struct KvProblem {
     int fp_layout;             // forward tile layout: 1, 2, 4 or 8 rows per wavefront
     int fp_tx, fp_ty;          // forward tiles per view (fp_tx padded to a multiple of KV_XCDS * fp_chunk)
     int fp_chunk;              // tile columns per XCD stripe
+    int fp_nch;                // stripes per tile row in one list: fp_tx / (KV_XCDS * fp_chunk)
+    KvUDiv fp_dchunk, fp_dnch, fp_dty;  // divisors fp_chunk, fp_nch, fp_ty
     long long fp_first;
     int bp_zt;                 // backprojection brick depth: 4 or 8 slices
     int bp_bx, bp_by, bp_bz;   // bricks per axis
