@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 (demangled) names of the two fused MLEM kernels
 FP_KERNEL = "fp_r_kernel<1>"
-BP_KERNEL = "bp_tiled_kernel<0, 1, 8, false>"
+BP_KERNEL = "bp_tiled_kernel<0, 1, 8, false, true>"
 METRIC = "time to RMSE≤1% vs converged MLEM, learned vs plain; GRays/s fwd+back"
 
 
