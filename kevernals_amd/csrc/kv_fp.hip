@@ -110,6 +110,15 @@ __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
     return r;
 }
 
+// a in the lanes whose bit of mask is set, else b: one v_cndmask with a
+// scalar mask (lanes outside the mask's active set are not written).
+__device__ __forceinline__ float selm(unsigned long long mask, float a, float b)
+{
+    float r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
+    return r;
+}
+
 // Minimum of three run heads as one v_min3_f32.  The heads are finite or
 // +INF (never NaN), so the IEEE-mode operand canonicalisation the compiler
 // would add for fminf is not needed.
@@ -132,15 +141,6 @@ __device__ __forceinline__ KvRsrc voxel_rsrc(const KvGeom &g, const float *vol)
     r.z = g.nx * g.ny * g.nz;                                   // records (at most 2^30)
     r.w = 0x00020000;
     return r;
-}
-
-// Raw buffer resource over the volume (byte range clamped to 32 bits; the
-// volume holds at most 2^30 voxels).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const KvGeom &g, const float *vol)
-{
-    const unsigned long long bytes = (unsigned long long)g.nx * g.ny * g.nz * 4ull;
-    const int n = bytes >= 0xFFFFFFFFull ? (int)0xFFFFFFFFu : (int)bytes;
-    return __builtin_amdgcn_make_buffer_rsrc((void *)vol, (short)0, n, 0x00020000);
 }
 
 // Mode-R walk of one ray.  visit(ok, voxel_index, weight) is called for every
@@ -227,24 +227,25 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     // segments are unchanged and one run's increment and end threshold are
     // selected instead of stepping all three.
     auto advance1 = [&](float m) {
-        // Two vector compares of bit patterns (the scalar unit combines the
-        // masks; a float compare's negation would be a third compare, as it
-        // must be true for NaN).  m is one of the three heads, so its bits
-        // equal those of the first run that holds it; heads that equal m as
-        // floats with other bits (+0 and -0) only order the consumption of
-        // equal list entries, which bound zero-width segments.
-        const bool px = __float_as_uint(ax) == __float_as_uint(m);
-        const bool qy = __float_as_uint(ay) == __float_as_uint(m);
-        // selects of values (a conditional of the captured lvalues would
-        // select their addresses and load through them)
-        const float vix = ix, viy = iy, viz = iz, vtx = tx, vty = ty, vtz = tz;
-        const float inc = px ? vix : (qy ? viy : viz);
-        const float thr = px ? vtx : (qy ? vty : vtz);
+        // Two vector compares of bit patterns give the lane masks of "x
+        // holds m" and "y holds m"; the scalar unit combines them into the
+        // three run masks and the selects take them directly (left to
+        // itself the compiler negates a mask with a third vector compare).
+        // m is one of the three heads, so its bits equal those of the first
+        // run that holds it; heads that equal m as floats with other bits
+        // (+0 and -0) only order the consumption of equal list entries,
+        // which bound zero-width segments.
+        const unsigned mb = __float_as_uint(m);
+        const unsigned long long mx = __builtin_amdgcn_ballot_w64(__float_as_uint(ax) == mb);
+        const unsigned long long my = __builtin_amdgcn_ballot_w64(__float_as_uint(ay) == mb);
+        const unsigned long long mpy = my & ~mx, mpz = ~(mx | my);
+        const float inc = selm(mx, ix, selm(my, iy, iz));
+        const float thr = selm(mx, tx, selm(my, ty, tz));
         float nv = m + inc;
         nv = nv > thr ? kInf : nv;
-        ax = px ? nv : ax;
-        ay = (!px && qy) ? nv : ay;
-        az = (!px && !qy) ? nv : az;
+        ax = selm(mx, nv, ax);
+        ay = selm(mpy, nv, ay);
+        az = selm(mpz, nv, az);
     };
     // a[i] = a[i-1] + |1/d| in FP32 (Projector.cu:156,177,199)
     auto advance = [&](float m) {
@@ -660,21 +661,25 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
     const int sx = (int)st[0], sy = (int)st[1] * nx, sz = (int)st[2] * nxy;
     unsigned idx = (unsigned)(ii[2] * nxy + ii[1] * nx + ii[0]);
     const int guard = r.n[0] + r.n[1] + r.n[2] + 3;
+    // max(0, lo[0], lo[1], lo[2]) of the current voxel.  The exiting axis's
+    // lo becomes its old hi, which is the step's minimum head and at least
+    // its old lo, so the new maximum is max(old maximum, that minimum): one
+    // running max, the same value as the maximum over the three axes (IEEE
+    // max of non-NaN values is associative).
+    float ent = fmaxf(0.f, max3f(lo[0], lo[1], lo[2]));
     for (int step = 0; step < guard; ++step) {
         const float best = min3f(hi[0], hi[1], hi[2]);
-        const float len = fminf(1.f, best) - fmaxf(0.f, max3f(lo[0], lo[1], lo[2]));
+        const float len = fminf(1.f, best) - ent;
         visit(len > 0.f, idx, len);
         if (best >= aout)
             break;
+        ent = max3f(ent, best, best);
         const bool e0 = hi[0] == best;
         const bool e1 = !e0 && hi[1] == best;
         const bool e2 = !e0 && !e1;
         idx += (unsigned)(e0 ? sx : (e1 ? sy : sz));
         // the exiting axis: its next plane (same operation sequence as plane_a)
         const float k0 = kf[0] + st[0], k1 = kf[1] + st[1], k2 = kf[2] + st[2];
-        lo[0] = e0 ? hi[0] : lo[0];
-        lo[1] = e1 ? hi[1] : lo[1];
-        lo[2] = e2 ? hi[2] : lo[2];
         hi[0] = e0 ? (k0 - r.s[0]) * r.inv[0] : hi[0];
         hi[1] = e1 ? (k1 - r.s[1]) * r.inv[1] : hi[1];
         hi[2] = e2 ? (k2 - r.s[2]) * r.inv[2] : hi[2];
@@ -686,29 +691,24 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
 
 // Sum over a mode-A walk of len * vol[voxel] for the accepted segments, the
 // value of acc = ok ? acc + len * v : acc, with no branch around the load:
-// a rejected segment loads at the volume's end (zero) with weight zero, and
-// each product is added one step later, so that the load is in flight
-// while the next step is walked.  acc starts at +0 and never becomes -0,
-// so adding a rejected segment's +0 leaves it unchanged: the sum is the
-// select form's bit for bit (for finite volumes; a volume of exactly 2^30
-// voxels reads voxel 0 for rejected segments, again times zero).
+// every step gathers its voxel (the walk's indices are always in range) and
+// a rejected segment carries weight +0, whose legacy product is +-0 for any
+// voxel value (INF and NaN included) and leaves acc unchanged (acc starts
+// at +0 and never becomes -0); an accepted weight is positive, where the
+// legacy product is the IEEE one.  Each product is added one step later, so
+// that the load is in flight while the next step is walked.
 struct AccA {
-    __amdgpu_buffer_rsrc_t rs;
-    unsigned oob;  // first index past the volume
+    KvRsrc rs;
     float acc = 0.f, pw = 0.f, pv = 0.f;
-    __device__ __forceinline__ void init(const KvGeom &g, const float *vol)
-    {
-        rs = volume_rsrc(g, vol);
-        oob = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
-    }
+    __device__ __forceinline__ void init(const KvGeom &g, const float *vol) { rs = voxel_rsrc(g, vol); }
     __device__ __forceinline__ void visit(bool ok, unsigned k, float len)
     {
-        const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((ok ? k : oob) << 2), 0, 0));
-        acc = acc + pw * pv;
+        const float v = kv_struct_load_f32(rs, (int)k, 0, 0, 0);
+        acc = acc + kv_mul_legacy(pw, pv);
         pw = ok ? len : 0.f;
         pv = v;
     }
-    __device__ __forceinline__ float sum() const { return acc + pw * pv; }
+    __device__ __forceinline__ float sum() const { return acc + kv_mul_legacy(pw, pv); }
 };
 
 // One ray of the mode-A forward pass with its epilogue.
