@@ -186,7 +186,7 @@ kv_status ensure_tables(kv_context *ctx, int mode)
             return KV_OK;
         KvTables &t = ctx->tab_r;
         kv_status st;
-        if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
+        if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4 + 16)) ||
             (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4)) ||
             (st = dev_alloc((void **)&t.yz, nyz * 4))) {
             free_tables(t);
@@ -200,7 +200,7 @@ kv_status ensure_tables(kv_context *ctx, int mode)
         return KV_OK;
     KvTables &t = ctx->tab_a;
     kv_status st;
-    if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4)) ||
+    if ((st = dev_alloc((void **)&t.x, nxe * 4)) || (st = dev_alloc((void **)&t.y, nye * 4 + 16)) ||
         (st = dev_alloc((void **)&t.z, nze * 4)) || (st = dev_alloc((void **)&t.lx2, (size_t)g.np * g.nu * 4)) ||
         (st = dev_alloc((void **)&t.ly2, (size_t)g.np * g.nv * 4)) ||
         (st = dev_alloc((void **)&t.lz2, (size_t)g.np * 4)) || (st = dev_alloc((void **)&t.zs, nze * KV_ZS * 4)) ||

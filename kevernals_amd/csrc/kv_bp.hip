@@ -288,6 +288,7 @@ struct BpShared {
 typedef const __attribute__((address_space(4))) float KvConstF;
 __device__ __forceinline__ KvConstF *kv_const(const float *p) { return (KvConstF *)(size_t)p; }
 
+
 // IEEE min/max of two floats that are results of arithmetic (never
 // signalling NaNs), without the operand quieting fminf/fmaxf add.
 __device__ __forceinline__ float vmin(float a, float b)
@@ -562,13 +563,11 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                             dst[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prs, off + 4 * c, 0, 0));
                     }
                 };
-                auto row = [&](int py, const float (&v)[NC]) {
-                    // the wavefront's y-plane crossings of this row (uniform:
-                    // scalar loads and scalar min/max)
-                    const float a1y = ty0[py], a2y = ty1[py];
-                    // max/min are exact: the reference's grouping
-                    // fmax(fmax(0,mx),fmax(my,mz)) regroups freely
-                    const float ly_ = bmin(a1y, a2y), hy_ = bmax(a1y, a2y);
+                // ly_, hy_: the row's y interval from the wavefront's two
+                // y-plane crossings (uniform: scalar min/max; exact, as the
+                // reference's grouping fmax(fmax(0,mx),fmax(my,mz)) regroups
+                // freely)
+                auto row = [&](int py, const float (&v)[NC], float ly_, float hy_) {
                     const float lyy = (MODE == KV_MODE_A && UNIT) ? ly2[py] : 0.f;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
@@ -597,16 +596,44 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                         }
                     }
                 };
+                // The y-plane crossings of a row pair are loaded one pair
+                // ahead (row indices clamped to the last row, so every load
+                // stays in the table): the pair's scalar loads complete
+                // while the previous pair is reduced, and the wait that the
+                // first row of a pair needs finds them done.
+                // The y intervals of a row pair come from scalar loads
+                // issued one pair ahead, at the top of the previous pair's
+                // turn, and reduced to (min, max) at its end, so the loads
+                // complete while that pair is reduced (kept in place by
+                // scheduling barriers: the loads of constant memory would
+                // otherwise be re-issued where their values are used).  Row
+                // hyw + 1 <= nv may be read: it lies in the table, or in the
+                // 16 bytes of padding after it, and is never used.
+                auto yint = [&](int r, float &lo, float &hi) {
+                    const float a = ty0[r], b = ty1[r];
+                    lo = bmin(a, b);
+                    hi = bmax(a, b);
+                };
                 load_row(ly, va);
                 int py = ly;
+                float lya, hya, lyb, hyb;
+                yint(py, lya, hya);
+                yint(py + 1, lyb, hyb);
                 for (; py < hyw; py += 2) {
+                    const float n0a = ty0[py + 2], n1a = ty1[py + 2], n0b = ty0[py + 3], n1b = ty1[py + 3];
+                    __builtin_amdgcn_sched_barrier(0);
                     load_row(py + 1, vb);
-                    row(py, va);
+                    row(py, va, lya, hya);
                     load_row(py + 2, va);
-                    row(py + 1, vb);
+                    row(py + 1, vb, lyb, hyb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    lya = bmin(n0a, n1a);
+                    hya = bmax(n0a, n1a);
+                    lyb = bmin(n0b, n1b);
+                    hyb = bmax(n0b, n1b);
                 }
                 if (py == hyw)
-                    row(py, va);
+                    row(py, va, lya, hya);
             };
             auto rows_nc = [&](auto simple_tag) {
                 if (__all(ncol <= 2))

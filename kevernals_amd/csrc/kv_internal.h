@@ -71,7 +71,7 @@ struct KvFpMap {
 // components of each ray (lx2[p][u], ly2[p][v], lz2[p]).
 struct KvTables {
     float *x = nullptr;   // [np][nx+1][nu]
-    float *y = nullptr;   // [np][ny+1][nv]
+    float *y = nullptr;   // [np][ny+1][nv], then 16 bytes of padding (read, never used)
     float *z = nullptr;   // [np][nz+1]
     float *lx2 = nullptr; // mode A: [np][nu]
     float *ly2 = nullptr; // mode A: [np][nv]
