@@ -102,11 +102,12 @@ __device__ __forceinline__ void axis_range(float d, float s, float h, int n, flo
 }
 
 // a * b + c on the low 24 bits of a and b, one instruction (the compiler
-// would otherwise form two products and a three-way add).
+// would otherwise form two products and a three-way add); b is
+// wavefront-uniform (a scalar operand).
 __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
 {
     unsigned r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
     return r;
 }
 
@@ -312,17 +313,24 @@ __device__ __forceinline__ void walk_r(const KvGeom &g, const RayF &r, Visit &&v
     const int left = (below(cx, ax) + below(cy, ay) + below(cz, az)) - ((ax == prev) + (ay == prev) + (az == prev));
     advance(prev);
     if (counted) {
-        int k = left;
-        for (; k >= 2; k -= 2) {
-            const float c1 = min3f(ax, ay, az);
-            advance1(c1);
-            segment(prev, c1);
-            const float c2 = min3f(ax, ay, az);
-            advance1(c2);
-            segment(c1, c2);
-            prev = c2;
+        // pairs of steps under a wavefront-uniform turn counter (the
+        // per-lane count only masks lanes), then the odd step
+        const int pairs = left >> 1;
+        for (int it = 0;; ++it) {
+            const bool act = it < pairs;
+            if (!__any(act))
+                break;
+            if (act) {
+                const float c1 = min3f(ax, ay, az);
+                advance1(c1);
+                segment(prev, c1);
+                const float c2 = min3f(ax, ay, az);
+                advance1(c2);
+                segment(c1, c2);
+                prev = c2;
+            }
         }
-        if (k > 0) {
+        if (left & 1) {
             const float c1 = min3f(ax, ay, az);
             advance1(c1);
             segment(prev, c1);
