@@ -219,7 +219,7 @@ __device__ __forceinline__ void shadow_affine(float c, float k1a, float k0a, flo
     hi = min((int)floorf(b), n - 1);
 }
 
-// Per view, z slice and y row: the terms of the backprojector that the
+// Per view, y row and z slice: the terms of the backprojector that the
 // voxels of one (y, z) row share (KvTables::yz), with the expressions the
 // backprojector evaluated per row before they were tabulated, so every value
 // is the same: the detector-row range of the voxel box's shadow between the
@@ -236,9 +236,9 @@ __global__ void yz_kernel(KvGeom g, KvTables t)
     const int half = 5;
     const float wy = (float)half * g.pv / g.sy;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int y = (int)(i % g.ny);
-        const int64_t r = i / g.ny;
-        const int z = (int)(r % g.nz), p = (int)(r / g.nz);
+        const int z = (int)(i % g.nz);
+        const int64_t r = i / g.nz;
+        const int y = (int)(r % g.ny), p = (int)(r / g.ny);
         const float *S = g.src + 3 * p;
         const float *D = g.det + 3 * p;
         const float sfy = S[1] / g.sy, ofy = D[1] / g.sy;
@@ -429,7 +429,7 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
         // (y, z)-row terms: uniform addresses in memory this kernel does not
         // write, read through the constant address space (scalar loads)
         const KvConstF *ty0 = kv_const(ty + (size_t)y * g.nv), *ty1 = kv_const(ty + (size_t)(y + 1) * g.nv);
-        const KvConstF *YZ = kv_const(t.yz + (((size_t)p * g.nz + z0) * g.ny + y) * KV_YZ);
+        const KvConstF *YZ = kv_const(t.yz + (((size_t)p * g.ny + y) * g.nz + z0) * KV_YZ);
         // Detector column of the voxel's lateral edges x = cvx and cvx + 1
         // on the plane at the top of slice zi (affine map of the per-view
         // table, bounds only); the bottom plane of slice zi is the top plane
@@ -445,11 +445,12 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             ua1 = ub1;
             // the wavefront's detector rows (empty when mode R's voxel
             // centre row lies off the widened detector) and z-plane terms
-            const KvConstF *E = YZ + (size_t)zi * g.ny * KV_YZ;
+            const KvConstF *E = YZ + zi * KV_YZ;
             const int ly = __float_as_int(E[0]), hyw = __float_as_int(E[1]);
+            const bool has_shadow = __float_as_int(E[2]) != 0;
+            const float lz = E[4], uz = E[5], lzp = E[6];  // one 8-word scalar load with the above
             if (ly > hyw)
                 continue;
-            const bool has_shadow = __float_as_int(E[2]) != 0;
             // Shadow of the voxel box on the detector columns: the columns
             // whose centres lie in [mnx - pad, mxx + pad] (see shadow_affine;
             // floor(a + 1) for ceil(a) only drops a column lying exactly on
@@ -502,7 +503,6 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
             }
             if (lx > hxw)
                 continue;
-            const float lz = E[4], uz = E[5], lzp = E[6];
             // one running sum per voxel across all views, as the reference
             // accumulates (Projector.cu:727-728)
             float a = accs[zi][0][tid], wsm = accs[zi][1][tid];

@@ -84,7 +84,8 @@ struct KvTables {
     //          coordinate c on plane l, u ~= Kx0 + c * Kx1 (bounds only)
     //  [4] Ky1, [5] Ky0: the same for detector rows
     float *zs = nullptr;
-    // Per view, z slice l and y row j (index (p * nz + l) * ny + j), KV_YZ
+    // Per view, y row j and z slice l (index (p * ny + j) * nz + l: the
+    // slices of one row are adjacent, as a brick reads them), KV_YZ
     // words, the backprojectors' terms shared by the voxels of one (y, z)
     // row: [0] first and [1] last detector row of the voxel box's shadow
     // (intersected with the +-5 window around the voxel centre's row in mode
