@@ -610,9 +610,10 @@ __device__ __forceinline__ void bp_brick(const KvGeom &g, const KvTables &t, con
                 // turn, and reduced to (min, max) at its end, so the loads
                 // complete while that pair is reduced (kept in place by
                 // scheduling barriers: the loads of constant memory would
-                // otherwise be re-issued where their values are used).  Row
-                // hyw + 1 <= nv may be read: it lies in the table, or in the
-                // 16 bytes of padding after it, and is never used.
+                // otherwise be re-issued where their values are used).  Rows
+                // up to hyw + 2 <= nv + 1 may be read: they lie in the table,
+                // or in the 16 bytes of padding after it, and are never
+                // used.
                 auto yint = [&](int r, float &lo, float &hi) {
                     const float a = ty0[r], b = ty1[r];
                     lo = bmin(a, b);

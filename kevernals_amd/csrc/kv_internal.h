@@ -48,6 +48,8 @@ struct KvUDiv {
 };
 inline KvUDiv kv_udiv(unsigned d)
 {
+    if (d < 1)
+        d = 1;  // no divisor of a valid launch is 0; keep the host arithmetic defined
     unsigned l = 0;
     while (l < 31 && (1ull << l) < d)
         ++l;
