@@ -237,6 +237,7 @@ void release_geometry(kv_context *ctx)
     free_dev(ctx->d_ratio);
     free_dev(ctx->d_sens);
     free_dev(ctx->d_ll);
+    free_dev(ctx->d_pack);
     ctx->tab_r_ok = ctx->tab_a_ok = ctx->sens_ok = false;
     ctx->has_geom = false;
 }
@@ -821,8 +822,12 @@ kv_status kv_log_likelihoods(kv_context *ctx, const float *d_data, const float *
         if (kv_status st = dev_alloc((void **)&ctx->d_ll, (KV_LL_MAX - 1) * nray * sizeof(float)))
             return st;
     }
+    if (n > 1 && !ctx->d_pack) {
+        if (kv_status st = dev_alloc((void **)&ctx->d_pack, 4 * (size_t)kv_nvox(ctx->g) * sizeof(float)))
+            return st;
+    }
     float *outs[KV_LL_MAX] = {ctx->d_ratio, ctx->d_ll, ctx->d_ll ? ctx->d_ll + nray : nullptr};
-    KV_TRY_HIP(kv_launch_fp_a_multi(ctx->g, n, d_vols, outs, ctx->stream));
+    KV_TRY_HIP(kv_launch_fp_a_multi(ctx->g, n, d_vols, ctx->d_pack, outs, ctx->stream));
     std::vector<double> part(nb);
     for (int i = 0; i < n; ++i) {
         // the same reduction as kv_log_likelihood

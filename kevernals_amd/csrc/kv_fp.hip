@@ -31,8 +31,14 @@
 // rsrc, record index, byte offset within the record, scalar offset, cache
 // policy.
 typedef int KvRsrc __attribute__((ext_vector_type(4)));
+typedef float KvF32x2 __attribute__((ext_vector_type(2)));
+typedef float KvF32x4 __attribute__((ext_vector_type(4)));
 __device__ float kv_struct_load_f32(KvRsrc rsrc, int vindex, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.struct.buffer.load.f32");
+__device__ KvF32x2 kv_struct_load_f32x2(KvRsrc rsrc, int vindex, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.struct.buffer.load.v2f32");
+__device__ KvF32x4 kv_struct_load_f32x4(KvRsrc rsrc, int vindex, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.struct.buffer.load.v4f32");
 
 namespace {
 
@@ -133,13 +139,13 @@ __device__ __forceinline__ float min3f(float a, float b, float c)
 // Volume gathers by voxel index: a structured buffer of 4-byte records,
 // bounds-checked by index (an index at or past the voxel count reads 0), so
 // the gather takes the linear index with no byte-offset shift.
-__device__ __forceinline__ KvRsrc voxel_rsrc(const KvGeom &g, const float *vol)
+__device__ __forceinline__ KvRsrc voxel_rsrc(const KvGeom &g, const float *vol, unsigned stride = 4)
 {
     const unsigned long long a = (unsigned long long)vol;
     KvRsrc r;
     r.x = (int)(unsigned)a;
-    r.y = (int)(((unsigned)(a >> 32) & 0xFFFFu) | (4u << 16));  // record stride 4 bytes
-    r.z = g.nx * g.ny * g.nz;                                   // records (at most 2^30)
+    r.y = (int)(((unsigned)(a >> 32) & 0xFFFFu) | (stride << 16));  // record stride in bytes
+    r.z = g.nx * g.ny * g.nz;                                        // records (at most 2^30)
     r.w = 0x00020000;
     return r;
 }
@@ -761,16 +767,71 @@ __global__ __launch_bounds__(256) void fp_a_kernel(KvGeom g, KvTables t, const f
     fp_a_ray<EP>(g, t, vol, out, data, p, u, v);
 }
 
-// Mode-A forward projections of NV volumes in one walk per ray: each
-// volume's sum takes the same products and additions in the same order as
-// fp_a_ray, so every projection is bit-identical to a single-volume pass.
+// Mode-A forward projections of NV <= 3 volumes in one walk per ray: one
+// volume gathered as fp_a_ray does, two or three interleaved per voxel
+// (pack_kernel: records of 2 or 4 floats) so that one gather per step reads
+// them all.  Each volume's sum takes the same products and additions in the
+// same order as fp_a_ray, so every projection is bit-identical to a
+// single-volume pass.
 struct KvMulti {
     const float *vol[KV_LL_MAX];
     float *out[KV_LL_MAX];
 };
 
+// The sums of a walk over NV = 2 or 3 interleaved volumes, as AccA's.
 template <int NV>
-__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, KvFpMap fm)
+struct AccPack {
+    static constexpr int W = NV == 2 ? 2 : 4;  // floats per record
+    KvRsrc rs;
+    float acc[NV], pv[NV];
+    float pw = 0.f;
+    __device__ __forceinline__ void init(const KvGeom &g, const float *pack)
+    {
+        rs = voxel_rsrc(g, pack, 4 * W);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            acc[i] = pv[i] = 0.f;
+    }
+    __device__ __forceinline__ void visit(bool ok, unsigned k, float len)
+    {
+        float v[NV];
+        if constexpr (NV == 2) {
+            const KvF32x2 r = kv_struct_load_f32x2(rs, (int)k, 0, 0, 0);
+            v[0] = r.x, v[1] = r.y;
+        } else {
+            const KvF32x4 r = kv_struct_load_f32x4(rs, (int)k, 0, 0, 0);
+            v[0] = r.x, v[1] = r.y, v[2] = r.z;
+        }
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            acc[i] = acc[i] + kv_mul_legacy(pw, pv[i]);
+        pw = ok ? len : 0.f;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            pv[i] = v[i];
+    }
+    __device__ __forceinline__ float sum(int i) const { return acc[i] + kv_mul_legacy(pw, pv[i]); }
+};
+
+// Volumes m.vol[0..NV) interleaved per voxel: float2 records for two, float4
+// records (fourth float 0) for three.
+template <int NV>
+__global__ __launch_bounds__(256) void pack_kernel(KvMulti m, int64_t n, float *__restrict__ pack)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        if constexpr (NV == 2) {
+            const float2 r = make_float2(m.vol[0][v], m.vol[1][v]);
+            reinterpret_cast<float2 *>(pack)[v] = r;
+        } else {
+            const float4 r = make_float4(m.vol[0][v], m.vol[1][v], m.vol[2][v], 0.f);
+            reinterpret_cast<float4 *>(pack)[v] = r;
+        }
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, const float *__restrict__ pack, KvMulti m,
+                                                         KvFpMap fm)
 {
     int tx, ty, p, u, v;
     fp_xcd_tile(fm, tx, ty, p);
@@ -779,19 +840,20 @@ __global__ __launch_bounds__(256) void fp_a_multi_kernel(KvGeom g, KvMulti m, Kv
         return;
     RayA r;
     ray_setup_a(g, p, u, v, r);
-    AccA a[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-        a[i].init(g, m.vol[i]);
-    walk_a(r, [&](bool ok, unsigned k, float len) {
+    const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
+    if constexpr (NV == 1) {
+        AccA a;
+        a.init(g, m.vol[0]);
+        walk_a(r, [&](bool ok, unsigned k, float len) { a.visit(ok, k, len); });
+        m.out[0][o] = r.L * a.sum();
+    } else {
+        AccPack<NV> a;
+        a.init(g, pack);
+        walk_a(r, [&](bool ok, unsigned k, float len) { a.visit(ok, k, len); });
 #pragma unroll
         for (int i = 0; i < NV; ++i)
-            a[i].visit(ok, k, len);
-    });
-    const size_t o = ((size_t)p * g.nv + v) * g.nu + u;
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-        m.out[i][o] = r.L * a[i].sum();
+            m.out[i][o] = r.L * a.sum(i);
+    }
 }
 
 // Persistent mode-A forward pass (MLEM ratio) over a batch, as
@@ -907,18 +969,29 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
     });
 }
 
-hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *const *outs, hipStream_t s)
+hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *pack, float *const *outs,
+                                hipStream_t s)
 {
     KvMulti m{};
     for (int i = 0; i < n; ++i)
         m.vol[i] = vols[i], m.out[i] = outs[i];
+    if (n >= 2) {
+        const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+        const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (nvox + 255) / 256));
+        if (n == 2)
+            pack_kernel<2><<<nb, 256, 0, s>>>(m, nvox, pack);
+        else
+            pack_kernel<3><<<nb, 256, 0, s>>>(m, nvox, pack);
+        if (hipError_t e = hipGetLastError())
+            return e;
+    }
     const int L = KV_FP_LAYOUT_MULTI[std::min(std::max(n, 1), KV_LL_MAX) - 1];
     const FpGrid fg(g, L);
     return fg.run(g, [&](unsigned nb, int p0) {
         switch (n) {
-        case 1: fp_a_multi_kernel<1><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
-        case 2: fp_a_multi_kernel<2><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
-        default: fp_a_multi_kernel<3><<<nb, 256, 0, s>>>(g, m, fg.map(p0)); break;
+        case 1: fp_a_multi_kernel<1><<<nb, 256, 0, s>>>(g, pack, m, fg.map(p0)); break;
+        case 2: fp_a_multi_kernel<2><<<nb, 256, 0, s>>>(g, pack, m, fg.map(p0)); break;
+        default: fp_a_multi_kernel<3><<<nb, 256, 0, s>>>(g, pack, m, fg.map(p0)); break;
         }
     });
 }

@@ -150,6 +150,7 @@ struct kv_context {
     double *d_red = nullptr;   // partial sums of deterministic reductions
     size_t red_cap = 0;        // doubles in d_red
     float *d_ll = nullptr;     // projections of the further volumes of kv_log_likelihoods
+    float *d_pack = nullptr;   // kv_log_likelihoods' volumes interleaved per voxel (4 floats per voxel)
 };
 
 // Reconstruction count and operator mode of a batch (0 / -1 for null).
@@ -206,13 +207,16 @@ inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 // kernels' default layouts: mode R 8, mode A 4 (eight full-size mode-A
 // reconstructions: 3.08 s to 1 % RMSE against 3.12 s in 8x8); the autotuner
 // may choose others.
-// The multi-volume likelihood walk takes its layout by volume count (one,
-// two, three gathers per step): 8.5 / 14.1 / 21.7 ms in 8x8 / 16x4 / 32x2
-// rays per wavefront, against 10.4 / 15.7 / 25.2 ms in 64x1 and 8.5 / 17.9 /
-// 31.0 ms in 8x8 for all counts.
+// The multi-volume likelihood walk takes its layout by volume count: one
+// gather per step, of one volume or of a record holding two or three
+// (interleaved first by pack_kernel).  Measured at the size above (walk,
+// reductions and packing per call): 7.5 / 9.4 / 11.7 ms for one / two /
+// three volumes in 8x8 / 16x4 / 16x4 rays per wavefront; two and three
+// volumes took 10.3 / 13.4 ms in 8x8, three 13.6 ms in 32x2, and one gather
+// per volume and step 12.5 / 21.5 ms.
 constexpr int KV_XCDS = 8;
 constexpr int KV_FP_LAYOUT = 8;
-constexpr int KV_FP_LAYOUT_MULTI[KV_LL_MAX] = {8, 4, 2};
+constexpr int KV_FP_LAYOUT_MULTI[KV_LL_MAX] = {8, 4, 4};
 constexpr int KV_FP_STRIPE = 32;
 constexpr int KV_FP_BATCH_LAYOUT_R = 8;
 constexpr int KV_FP_BATCH_LAYOUT_A = 4;
@@ -223,7 +227,10 @@ hipError_t kv_launch_fp_r(const KvGeom &g, const float *vol, float *out, int epi
 // premultiplied by the ray's length factor from the mode-A tables t (the
 // input the mode-A backprojector expects); the other epilogues ignore t.
 // Mode-A forward projections of n <= KV_LL_MAX volumes in one walk per ray.
-hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *const *outs, hipStream_t s);
+// For n >= 2 the volumes are first interleaved into pack (4 floats per
+// voxel), so that one gather per step reads every volume.
+hipError_t kv_launch_fp_a_multi(const KvGeom &g, int n, const float *const *vols, float *pack, float *const *outs,
+                                hipStream_t s);
 hipError_t kv_launch_fp_a(const KvGeom &g, const KvTables &t, const float *vol, float *out, int epilogue,
                           const float *data, hipStream_t s);
 hipError_t kv_launch_trace(const KvGeom &g, int mode, int nrays, const int32_t *rays, int cap, int64_t *idx,
