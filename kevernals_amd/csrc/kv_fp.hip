@@ -126,6 +126,14 @@ __device__ __forceinline__ float selm(unsigned long long mask, float a, float b)
     return r;
 }
 
+// min(1, a) as one instruction (a is never NaN where it is used).
+__device__ __forceinline__ float min1(float a)
+{
+    float r;
+    asm("v_min_f32 %0, 1.0, %1" : "=v"(r) : "v"(a));
+    return r;
+}
+
 // Minimum of three run heads as one v_min3_f32.  The heads are finite or
 // +INF (never NaN), so the IEEE-mode operand canonicalisation the compiler
 // would add for fminf is not needed.
@@ -681,25 +689,39 @@ __device__ __forceinline__ void walk_a(const RayA &r, Visit &&visit)
     // running max, the same value as the maximum over the three axes (IEEE
     // max of non-NaN values is associative).
     float ent = fmaxf(0.f, max3f(lo[0], lo[1], lo[2]));
-    for (int step = 0; step < guard; ++step) {
+    // One step: visit the segment through the current voxel, then true at
+    // the walk's end, else move to the next voxel.  The exit axis's lane
+    // masks come from two vector compares (the scalar unit forms the first-
+    // minimum masks), and the selects take them directly.  Two steps per
+    // turn of the guard loop: the guard only bounds the walk (a ray crosses
+    // fewer planes than it counts), so one test per two steps will do.
+    auto step = [&]() {
         const float best = min3f(hi[0], hi[1], hi[2]);
-        const float len = fminf(1.f, best) - ent;
+        const float len = min1(best) - ent;
         visit(len > 0.f, idx, len);
         if (best >= aout)
-            break;
+            return true;
         ent = max3f(ent, best, best);
-        const bool e0 = hi[0] == best;
-        const bool e1 = !e0 && hi[1] == best;
-        const bool e2 = !e0 && !e1;
-        idx += (unsigned)(e0 ? sx : (e1 ? sy : sz));
+        const unsigned long long m0 = __builtin_amdgcn_ballot_w64(hi[0] == best);
+        const unsigned long long m1 = __builtin_amdgcn_ballot_w64(hi[1] == best);
+        const unsigned long long p1 = m1 & ~m0, p2 = ~(m0 | m1);
+        idx += (unsigned)__float_as_int(
+            selm(m0, __int_as_float(sx), selm(p1, __int_as_float(sy), __int_as_float(sz))));
         // the exiting axis: its next plane (same operation sequence as plane_a)
         const float k0 = kf[0] + st[0], k1 = kf[1] + st[1], k2 = kf[2] + st[2];
-        hi[0] = e0 ? (k0 - r.s[0]) * r.inv[0] : hi[0];
-        hi[1] = e1 ? (k1 - r.s[1]) * r.inv[1] : hi[1];
-        hi[2] = e2 ? (k2 - r.s[2]) * r.inv[2] : hi[2];
-        kf[0] = e0 ? k0 : kf[0];
-        kf[1] = e1 ? k1 : kf[1];
-        kf[2] = e2 ? k2 : kf[2];
+        hi[0] = selm(m0, (k0 - r.s[0]) * r.inv[0], hi[0]);
+        hi[1] = selm(p1, (k1 - r.s[1]) * r.inv[1], hi[1]);
+        hi[2] = selm(p2, (k2 - r.s[2]) * r.inv[2], hi[2]);
+        kf[0] = selm(m0, k0, kf[0]);
+        kf[1] = selm(p1, k1, kf[1]);
+        kf[2] = selm(p2, k2, kf[2]);
+        return false;
+    };
+    for (int n = 0; n < guard; n += 2) {
+        if (step())
+            break;
+        if (step())
+            break;
     }
 }
 
