@@ -122,7 +122,8 @@ __global__ __launch_bounds__(256) void traj_gram_kernel(KvTraj tr, int h, int64_
             load_history<HB>(tr, h, v, x);
             const float xh = x[HB];
             const float s = sens[v];
-            const float sw = (s > 0.f && xh > 0.f) ? sqrtf(s * xh) : 0.f;
+            // sqrt(w) = sqrt(s) sqrt(x_h): no overflow of the product near the top of the range
+            const float sw = (s > 0.f && xh > 0.f) ? sqrtf(s) * sqrtf(xh) : 0.f;
             bool any = false;
 #pragma unroll
             for (int i = 0; i < HB; ++i) {

@@ -109,6 +109,74 @@ def test_combine_matches_oracle(K, run, h):
     np.testing.assert_array_equal(last.cpu().numpy(), out.cpu().numpy())
 
 
+def extreme_pair(shape, seed):
+    """Iterate pairs (x0, x1) covering every path of the passes' log-ratio:
+    ratios near 1 (the series; 0), around b/a = 3/5 and 5/3 (its edges; 1,
+    2), far from 1 (the hardware log of the quotient; 3), a subnormal or
+    barely normal x0 under a moderate x1 (its reciprocal leaves the normal
+    range; 4), magnitudes whose sum leaves it at the top (5), zero or
+    negative entries (ratio 0; 6), and both subnormal (7)."""
+    r = rng(seed)
+    n = int(np.prod(shape))
+    cat = r.integers(0, 8, n)
+    x0 = np.exp(r.uniform(-20.0, 20.0, n))
+    x1 = x0 * np.where(cat == 0, 1.0 + r.uniform(-1e-3, 1e-3, n), 1.0)
+    x1 = np.where(cat == 1, x0 * r.uniform(0.55, 0.65, n), x1)
+    x1 = np.where(cat == 2, x0 * r.uniform(1.6, 1.75, n), x1)
+    x1 = np.where(cat == 3, x0 * np.exp(r.uniform(-8.0, 8.0, n)), x1)
+    x0 = np.where(cat == 4, 10.0 ** r.uniform(-40.0, -36.0, n), x0)
+    x1 = np.where(cat == 4, 10.0 ** r.uniform(-6.0, -4.0, n), x1)  # d <= 83: exp(d) stays finite
+    x0 = np.where(cat == 5, r.uniform(1.2e38, 1.7e38, n), x0)
+    x1 = np.where(cat == 5, x0 * r.uniform(0.8, 1.25, n), x1)
+    x0 = np.where(cat == 7, r.uniform(1e-44, 1e-38, n), x0)
+    x1 = np.where(cat == 7, x0 * r.uniform(0.5, 2.0, n), x1)
+    x0 = x0.astype(np.float32)
+    x1 = x1.astype(np.float32)
+    zero = cat == 6
+    x0[zero & (r.random(n) < 0.5)] = 0.0
+    x1[zero & (r.random(n) < 0.5)] = -1.0
+    return x0.reshape(shape), x1.reshape(shape), cat.reshape(shape)
+
+
+def test_log_ratio_paths_against_oracle(K, run):
+    """Every path of the log-ratio (series, quotient log, log1p for extreme
+    magnitudes, zero) through both passes, against the FP64 oracle: the
+    one-ratio Gram entry to 1e-6, and the combination with c = 1 (y =
+    x1 * exp(d), d = log(x1 / x0)) per voxel to 2e-6 + 5e-7 |d| relative
+    (a few units in the last place of d, amplified by |d|) where y is a
+    normal float; everywhere finite, and x1 itself where a ratio is not
+    positive."""
+    from oracle import traj
+
+    ctx, _, its = run
+    shape = its[0].shape
+    x0, x1, cat = extreme_pair(shape, 91)
+    assert all(np.any(cat == c) for c in range(8))
+    xs = [torch.from_numpy(x0).cuda(), torch.from_numpy(x1).cuda()]
+    got = ctx.traj_gram(xs)
+    want = traj.gram([x0, x1], sensitivity(K, ctx))
+    print(f"[traj] extreme pairs: gram {got[0, 0]:.9e} vs {want[0, 0]:.9e}", flush=True)
+    assert abs(got[0, 0] - want[0, 0]) <= 1e-6 * want[0, 0]
+    out = torch.empty_like(xs[0])
+    ctx.traj_combine(xs, [1.0], out)
+    torch.cuda.synchronize()
+    y = out.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(y))
+    ref = traj.combine([x0, x1], [1.0])
+    d = traj.log_ratios([x0, x1])[0].reshape(shape)
+    f32 = np.finfo(np.float32)
+    ok = (np.abs(ref) >= f32.tiny) & (np.abs(ref) < f32.max)
+    rel = np.abs(y[ok] - ref[ok]) / np.abs(ref[ok])
+    bound = 2e-6 + 5e-7 * np.abs(d[ok])
+    worst = {c: float((rel / bound)[cat[ok] == c].max()) for c in range(8) if np.any(cat[ok] == c)}
+    print(f"[traj] extreme pairs: combine worst error / bound by path {worst}", flush=True)
+    assert np.all(rel <= bound)
+    for c in (0, 1, 2, 3, 4, 5):
+        assert np.any(ok & (cat == c)), f"path {c} unchecked"
+    keep = (x0 <= 0) | (x1 <= 0)
+    np.testing.assert_array_equal(out.cpu().numpy()[keep], x1[keep])
+
+
 def test_jump_along_mlem_lands_near_later_iterate(K):
     """History of MLEM-A iterations 6..20 on a 96x96x24 / 9-view problem,
     jump 40 iterations ahead: the estimate is several times closer to
