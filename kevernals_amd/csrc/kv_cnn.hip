@@ -42,12 +42,18 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (inputs are finite)
+// float -> bfloat16 bits, round to nearest even (inputs are finite): the
+// hardware conversion (v_cvt_pk_bf16_f32), one or two values at a time
+typedef __bf16 kv_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float kv_f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned int f2bf(float f)
 {
-    unsigned int b = __float_as_uint(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return b >> 16;
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
+}
+// (lo, hi) -> the bfloat16 pair lo | hi << 16
+__device__ __forceinline__ unsigned int f2bf2(float lo, float hi)
+{
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector((kv_f32x2){lo, hi}, kv_bf16x2));
 }
 
 __device__ __forceinline__ float bf2f(unsigned int h) { return __uint_as_float(h << 16); }
@@ -96,19 +102,19 @@ template <int EP>
 __device__ __forceinline__ void store4(const f32x4 &acc, int g, size_t v, const float *bias,
                                        unsigned short *out16, const unsigned short *mask)
 {
-    unsigned int o[4];
+    float o[4];
     if (EP == EP_RELU) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            o[i] = f2bf(fmaxf(acc[i] + bias[4 * g + i], 0.f));
+            o[i] = fmaxf(acc[i] + bias[4 * g + i], 0.f);
     } else {
         const uint2 m = *reinterpret_cast<const uint2 *>(mask + v * 16 + 4 * g);
         const unsigned int mb[4] = {m.x & 0xFFFFu, m.x >> 16, m.y & 0xFFFFu, m.y >> 16};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            o[i] = bf2f(mb[i]) > 0.f ? f2bf(acc[i]) : 0u;
+            o[i] = bf2f(mb[i]) > 0.f ? acc[i] : 0.f;
     }
-    *reinterpret_cast<uint2 *>(out16 + v * 16 + 4 * g) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+    *reinterpret_cast<uint2 *>(out16 + v * 16 + 4 * g) = make_uint2(f2bf2(o[0], o[1]), f2bf2(o[2], o[3]));
 }
 
 // Weight operands of every convolution, packed once per call as the MFMA A
