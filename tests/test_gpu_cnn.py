@@ -160,6 +160,29 @@ def test_forward_matches_training_pass_and_oracle(K, oracle_mod):
         ctx.cnn_forward(torch.from_numpy(params).cuda(), SCALE, xd, y, work[:16])
 
 
+@pytest.mark.parametrize("shape", [(9, 11, 21), (70, 9, 18)])
+def test_forward_z_chunks_against_oracle(K, oracle_mod, shape):
+    """Depths that exercise the convolution's z-walk: 9 slices (the last
+    4-slice chunk starts on the last slice, so its new halo planes all lie
+    past the volume) and 70 slices (a workgroup's 16 chunks end at slice 64
+    and a second workgroup starts a fresh halo ring)."""
+    from oracle import cnn
+
+    x, _, params = inputs(3, shape)
+    ctx = K.Context(0)
+    work = torch.empty(K.cnn_workspace_size(shape, False), dtype=torch.uint8, device="cuda")
+    y = torch.empty(shape, device="cuda")
+    ctx.cnn_forward(torch.from_numpy(params).cuda(), SCALE, torch.from_numpy(x).cuda(), y, work)
+    got = y.cpu().numpy().astype(np.float64)
+    ref_bf, _ = cnn.forward(params, x, SCALE, bf16_round=True)
+    corr = np.abs(ref_bf - x)
+    err = np.abs(got - ref_bf)
+    print(f"[cnn] forward {shape} vs bf16 oracle: max err {err.max():.3e} (max correction {corr.max():.3e})",
+          flush=True)
+    assert err.max() <= 5e-3 * corr.max()
+    assert np.median(err) <= 1e-5 * corr.max()
+
+
 def test_gradient_vs_fp64_network(K, oracle_mod):
     from oracle import cnn
 
