@@ -168,8 +168,9 @@ kv_status kv_log_likelihood(kv_context *ctx, const float *d_data, const float *d
 /* kv_log_likelihood of n <= KV_LL_MAX volumes (out[i] for d_vols[i], each
  * bit-identical to a kv_log_likelihood call) with one ray walk for all of
  * them: the likelihood gates compare up to three candidates.  Uses two more
- * projection-sized buffers of the context (allocated on first use).
- * Synchronous. */
+ * projection-sized buffers of the context and, for n >= 2, a buffer of 16
+ * bytes per voxel into which the volumes are interleaved (all allocated on
+ * first use and kept with the context).  Synchronous. */
 #define KV_LL_MAX 3
 kv_status kv_log_likelihoods(kv_context *ctx, const float *d_data, const float *const *d_vols, int n, double *out);
 
