@@ -205,7 +205,8 @@ inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 // wavefront in 32-column stripes, mode R forward 10.9 -> 8.8 ms and mode A
 // 10.0 -> 8.3 ms against 64x1 rays per wavefront round-robin.  The batch
 // kernels' default layouts: mode R 8, mode A 4 (eight full-size mode-A
-// reconstructions: 3.08 s to 1 % RMSE against 3.12 s in 8x8); the autotuner
+// reconstructions, 100 batched iterations: 9.56 s against 10.45 s in 8x8);
+// the autotuner
 // may choose others.
 // The multi-volume likelihood walk takes its layout by volume count: one
 // gather per step, of one volume or of a record holding two or three
