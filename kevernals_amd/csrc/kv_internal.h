@@ -206,15 +206,15 @@ inline int64_t kv_nray(const KvGeom &g) { return (int64_t)g.np * g.nv * g.nu; }
 // 10.0 -> 8.3 ms against 64x1 rays per wavefront round-robin.  The batch
 // kernels' default layouts: mode R 8, mode A 4 (eight full-size mode-A
 // reconstructions, 100 batched iterations: 9.56 s against 10.45 s in 8x8);
-// the autotuner
-// may choose others.
+// the autotuner may choose others.
 // The multi-volume likelihood walk takes its layout by volume count: one
 // gather per step, of one volume or of a record holding two or three
 // (interleaved first by pack_kernel).  Measured at the size above (walk,
-// reductions and packing per call): 7.5 / 9.4 / 11.7 ms for one / two /
-// three volumes in 8x8 / 16x4 / 16x4 rays per wavefront; two and three
-// volumes took 10.3 / 13.4 ms in 8x8, three 13.6 ms in 32x2, and one gather
-// per volume and step 12.5 / 21.5 ms.
+// reductions and packing per call): 6.9 / 7.7 / 10.1 ms for one / two /
+// three volumes in 8x8 / 16x4 / 16x4 rays per wavefront; before the current
+// mode-A walk, two and three volumes took 9.4 / 11.7 ms in these layouts,
+// 10.3 / 13.4 ms in 8x8, three 13.6 ms in 32x2, and one gather per volume
+// and step 12.5 / 21.5 ms.
 constexpr int KV_XCDS = 8;
 constexpr int KV_FP_LAYOUT = 8;
 constexpr int KV_FP_LAYOUT_MULTI[KV_LL_MAX] = {8, 4, 4};
